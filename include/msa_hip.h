@@ -1,0 +1,198 @@
+/*
+ * msa_hip.h -- C ABI of the MI355X (gfx950) all-pairs Needleman-Wunsch stage.
+ *
+ * This is the drop-in boundary that replaces the reference's CUDA layer on
+ * the pairwise path: the runtime wrapper of src/cuda.cu / src/cuda.cuh and the
+ * hybrid algorithm of src/pairwise/needleman/impl/hybrid.cu.  Everything is
+ * plain C: pointers, sizes and int status codes; no C++ exception crosses it.
+ * Every function returns MSA_OK (0) on success, otherwise a status whose text
+ * msa_error_string() gives; msa_last_error() holds the detail of the last
+ * failure on the calling thread.
+ *
+ * Data conventions (unchanged from the reference):
+ *   - residues are 5-bit codes, three per uint16 block at bit shifts 1, 6 and
+ *     11, bit 0 set on a block that holds padding (src/encoder.hpp:24-77,
+ *     src/encoder.cpp:76-98); code 0x18 is the end/padding code and 0x19 the
+ *     gap code;
+ *   - a sequence database is all blocks back to back plus, per sequence, its
+ *     offset and size in blocks (pairwise::database, src/pairwise/database.cuh:
+ *     27-91, database.cu:24-72);
+ *   - the scoring table is 25x25 indexed [row code][column code] with a linear
+ *     gap penalty (src/pairwise/pairwise.cuh:124-182); it is passed as int32
+ *     because every entry the reference ships is an integer;
+ *   - scores come out as int32 in reference pair order: pair c = i(i-1)/2 + j
+ *     for i > j (src/pairwise/pairwise.cpp:73-85), or in the order of an
+ *     explicit pair list.  They equal the reference's sequential float scores
+ *     exactly (src/pairwise/needleman/impl/sequential.cpp:29-95).
+ */
+#ifndef MSA_HIP_H
+#define MSA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------
+ * Replace cuda::exception / cuda::check (src/cuda.cuh:146-221, cuda.cu:64-71), which the
+ * reference compiles out in production builds; here every call is checked. */
+enum {
+    MSA_OK = 0,
+    MSA_ERR_INVALID_ARGUMENT = 1,
+    MSA_ERR_NO_DEVICE = 2,
+    MSA_ERR_OUT_OF_MEMORY = 3,
+    MSA_ERR_RUNTIME = 4,        /* a HIP runtime call failed */
+    MSA_ERR_GAP_CODE = 5,       /* a sequence holds the gap code 0x19 */
+    MSA_ERR_TABLE = 6,          /* table entry outside what the kernels support */
+    MSA_ERR_TOO_LARGE = 7,      /* an input exceeds a documented maximum */
+    MSA_ERR_BUSY = 8,           /* serving: admission refused, try again */
+    MSA_ERR_NOT_READY = 9,      /* serving: job has not finished */
+    MSA_ERR_NOT_FOUND = 10      /* serving: unknown job id */
+};
+
+typedef struct msa_stream_opaque* msa_stream;   /* a hipStream_t; NULL = null stream */
+typedef struct msa_event_opaque* msa_event;     /* a hipEvent_t */
+
+const char* msa_error_string(int status);
+const char* msa_last_error(void);
+int msa_version(void);
+
+/* ---- device (device::count/current/select/properties, src/cuda.cu:73-127,
+ *      device::free_memory, src/cuda.cuh:253-258) ----------------------------- */
+typedef struct {
+    char name[256];
+    char arch[64];              /* gcnArchName, e.g. "gfx950:sramecc+:xnack-" */
+    int compute_units;
+    int wavefront_size;
+    int clock_khz;
+    int max_threads_per_block;
+    size_t lds_per_block;       /* bytes of LDS one workgroup may use */
+    size_t lds_per_cu;          /* bytes of LDS per compute unit */
+    size_t global_mem;
+} msa_device_props;
+
+int msa_device_count(int* count);
+int msa_device_select(int device);
+int msa_device_current(int* device);
+int msa_device_props_get(int device, msa_device_props* props);
+int msa_mem_info(size_t* free_bytes, size_t* total_bytes);   /* device::free_memory, hybrid.cu:350 */
+int msa_device_sync(void);                                     /* cuda::barrier, cuda.cuh:441 */
+
+/* ---- memory (allocator::device / allocator::pinned, src/cuda.cu:42-62;
+ *      memory::copy / memory::set, src/cuda.cuh:396-430) ----------------------- */
+int msa_malloc(void** ptr, size_t bytes);
+int msa_free(void* ptr);
+int msa_malloc_host(void** ptr, size_t bytes);     /* pinned */
+int msa_free_host(void* ptr);
+int msa_memcpy(void* dst, const void* src, size_t bytes);   /* synchronous, any direction */
+int msa_memcpy_async(void* dst, const void* src, size_t bytes, msa_stream stream);
+int msa_memset(void* ptr, int value, size_t bytes);
+
+/* ---- streams and events (cuda::stream, src/cuda.cuh:279-330) ------------- */
+int msa_stream_create(msa_stream* stream);
+int msa_stream_destroy(msa_stream stream);
+int msa_stream_sync(msa_stream stream);
+int msa_event_create(msa_event* event);
+int msa_event_destroy(msa_event event);
+int msa_event_record(msa_event event, msa_stream stream);
+int msa_event_sync(msa_event event);
+int msa_event_elapsed_ms(msa_event start, msa_event stop, float* ms);
+
+/* ---- scoring-table registry (scoring_table::has/make/list,
+ *      src/pairwise/table.cu:254-330) ---------------------------------------- */
+int msa_table_count(void);
+const char* msa_table_name(int index);
+/* Writes the 625 entries and the penalty of the named table; MSA_ERR_NOT_FOUND
+ * for an unknown name (the reference throws "unknown pairwise scoring table"). */
+int msa_table_get(const char* name, int32_t* table625, int32_t* penalty);
+
+/* ---- Needleman-Wunsch ------------------------------------------------------ */
+
+/* Kernel families. */
+enum {
+    MSA_KERNEL_AUTO = 0,     /* batched for clean pairs, plain for the rest, long for very long pairs */
+    MSA_KERNEL_PLAIN = 1,    /* one pair per workgroup at a time, every pair */
+    MSA_KERNEL_BATCHED = 2,  /* persistent length-binned queue (plain covers pairs it cannot take) */
+    MSA_KERNEL_LONG = 3      /* multi-workgroup wavefront for every pair (plain covers the rest) */
+};
+
+typedef struct {
+    int kernel;              /* MSA_KERNEL_* */
+    int use_graph;           /* capture the launches of one execution in a hipGraph */
+    int lanes_per_pair;      /* batched: 16, 32 or 64; 0 = tuned/default */
+    int rows_per_lane;       /* batched: 8, 16 or 32; 0 = tuned/default */
+    int long_threshold;      /* pairs whose shorter side exceeds this many residues take the
+                                long kernel under MSA_KERNEL_AUTO; 0 = default */
+    const char* tuning_file; /* JSON written by the autotuner; NULL = built-in defaults */
+} msa_nw_options;
+
+typedef struct {
+    int64_t pairs;           /* pairs scored by one execution */
+    int64_t cells;           /* sum over pairs of unpadded L1 * L2 */
+    int64_t plain_pairs;     /* pairs routed to the plain kernel */
+    int64_t batched_pairs;   /* pairs routed to the batched kernel */
+    int64_t long_pairs;      /* pairs routed to the long kernel */
+    int32_t launches;        /* kernel launches per execution */
+    int32_t bins;            /* batched length bins */
+    int64_t device_bytes;    /* device memory the plan holds */
+} msa_nw_plan_info;
+
+typedef struct msa_nw_plan_opaque* msa_nw_plan;
+
+/*
+ * Builds an execution plan and uploads everything it needs to the current
+ * device: the sequence store, the scoring table, the binned work lists.
+ * Replaces hybrid.cu:262-429 (required_memory, make_input, load_input).
+ *   blocks/offset/size/nseq : host sequence database (offset and size in blocks)
+ *   pairs                   : host array of 2*npairs sequence indices (first, second)
+ *                             as the reference's pair union holds them, or NULL
+ *                             for all pairs in reference order (npairs must then
+ *                             be nseq*(nseq-1)/2)
+ *   table625, penalty       : host scoring table and gap penalty
+ * Sequences holding the gap code 0x19 are refused (MSA_ERR_GAP_CODE): the
+ * reference reads past its 25x25 table for them.
+ */
+int msa_nw_plan_create(msa_nw_plan* plan,
+                       const uint16_t* blocks, const int64_t* offset, const int32_t* size, int32_t nseq,
+                       const int32_t* pairs, int64_t npairs,
+                       const int32_t* table625, int32_t penalty,
+                       const msa_nw_options* options, msa_stream stream);
+
+/* Scores every pair of the plan into device memory d_out (npairs int32),
+ * asynchronously on `stream`.  Inputs are already resident. */
+int msa_nw_plan_execute(msa_nw_plan plan, int32_t* d_out, msa_stream stream);
+int msa_nw_plan_info_get(msa_nw_plan plan, msa_nw_plan_info* info);
+int msa_nw_plan_destroy(msa_nw_plan plan);
+
+/*
+ * One-shot form of the above with host input and host output (out: npairs
+ * int32), synchronous.  This is the shape of the reference's align()
+ * (hybrid.cu:403-429): copy in, score, copy out.
+ */
+int msa_nw_align(const uint16_t* blocks, const int64_t* offset, const int32_t* size, int32_t nseq,
+                 const int32_t* pairs, int64_t npairs,
+                 const int32_t* table625, int32_t penalty,
+                 const msa_nw_options* options, int32_t* out);
+
+/*
+ * The plain kernel on caller-owned device memory: one pair per workgroup at a
+ * time, reference orientation (rows = the sequence with more blocks), every
+ * padding rule of sequential.cpp:29-68.  d_pairs holds 2*npairs indices, or
+ * is NULL for all pairs in reference order; max_blocks bounds size[] (it
+ * sizes the strip-boundary workspace).  The library keeps that workspace per
+ * device (grown on demand, never shrunk), so calls on one device must not
+ * overlap.  Replaces align_kernel (hybrid.cu:222-252) and its launch
+ * (hybrid.cu:423).
+ */
+int msa_nw_plain(const uint16_t* d_blocks, const int64_t* d_offset, const int32_t* d_size, int32_t nseq,
+                 int32_t max_blocks, const int32_t* d_pairs, int64_t npairs,
+                 const int32_t* table625, int32_t penalty,
+                 int32_t* d_out, msa_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MSA_HIP_H */

@@ -1,0 +1,249 @@
+"""ctypes binding of libmsa_hip.so, the C ABI declared in include/msa_hip.h.
+
+The library is loaded from the package directory (built in-tree by
+`python -m msa_amd.build` or __graft_entry__.build()).  There is no fallback:
+if the library is missing or fails to load, every GPU entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libmsa_hip.so")
+
+MSA_OK = 0
+MSA_ERR_INVALID_ARGUMENT = 1
+MSA_ERR_NO_DEVICE = 2
+MSA_ERR_OUT_OF_MEMORY = 3
+MSA_ERR_RUNTIME = 4
+MSA_ERR_GAP_CODE = 5
+MSA_ERR_TABLE = 6
+MSA_ERR_TOO_LARGE = 7
+MSA_ERR_BUSY = 8
+MSA_ERR_NOT_READY = 9
+MSA_ERR_NOT_FOUND = 10
+
+KERNEL_AUTO = 0
+KERNEL_PLAIN = 1
+KERNEL_BATCHED = 2
+KERNEL_LONG = 3
+
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_sz = ctypes.c_size_t
+
+
+class DeviceProps(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 256), ("arch", ctypes.c_char * 64),
+                ("compute_units", ctypes.c_int), ("wavefront_size", ctypes.c_int),
+                ("clock_khz", ctypes.c_int), ("max_threads_per_block", ctypes.c_int),
+                ("lds_per_block", _sz), ("lds_per_cu", _sz), ("global_mem", _sz)]
+
+
+class NwOptions(ctypes.Structure):
+    _fields_ = [("kernel", ctypes.c_int), ("use_graph", ctypes.c_int),
+                ("lanes_per_pair", ctypes.c_int), ("rows_per_lane", ctypes.c_int),
+                ("long_threshold", ctypes.c_int), ("tuning_file", ctypes.c_char_p)]
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [("pairs", _i64), ("cells", _i64), ("plain_pairs", _i64), ("batched_pairs", _i64),
+                ("long_pairs", _i64), ("launches", _i32), ("bins", _i32), ("device_bytes", _i64)]
+
+
+# name -> (restype, argtypes); every symbol include/msa_hip.h declares.
+SIGNATURES = {
+    "msa_error_string": (ctypes.c_char_p, [ctypes.c_int]),
+    "msa_last_error": (ctypes.c_char_p, []),
+    "msa_version": (ctypes.c_int, []),
+    "msa_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+    "msa_device_select": (ctypes.c_int, [ctypes.c_int]),
+    "msa_device_current": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+    "msa_device_props_get": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(DeviceProps)]),
+    "msa_mem_info": (ctypes.c_int, [ctypes.POINTER(_sz), ctypes.POINTER(_sz)]),
+    "msa_device_sync": (ctypes.c_int, []),
+    "msa_malloc": (ctypes.c_int, [ctypes.POINTER(_vp), _sz]),
+    "msa_free": (ctypes.c_int, [_vp]),
+    "msa_malloc_host": (ctypes.c_int, [ctypes.POINTER(_vp), _sz]),
+    "msa_free_host": (ctypes.c_int, [_vp]),
+    "msa_memcpy": (ctypes.c_int, [_vp, _vp, _sz]),
+    "msa_memcpy_async": (ctypes.c_int, [_vp, _vp, _sz, _vp]),
+    "msa_memset": (ctypes.c_int, [_vp, ctypes.c_int, _sz]),
+    "msa_stream_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
+    "msa_stream_destroy": (ctypes.c_int, [_vp]),
+    "msa_stream_sync": (ctypes.c_int, [_vp]),
+    "msa_event_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
+    "msa_event_destroy": (ctypes.c_int, [_vp]),
+    "msa_event_record": (ctypes.c_int, [_vp, _vp]),
+    "msa_event_sync": (ctypes.c_int, [_vp]),
+    "msa_event_elapsed_ms": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_float)]),
+    "msa_table_count": (ctypes.c_int, []),
+    "msa_table_name": (ctypes.c_char_p, [ctypes.c_int]),
+    "msa_table_get": (ctypes.c_int, [ctypes.c_char_p, _vp, ctypes.POINTER(_i32)]),
+    "msa_nw_plan_create": (ctypes.c_int, [ctypes.POINTER(_vp), _vp, _vp, _vp, _i32, _vp, _i64, _vp, _i32,
+                                          ctypes.POINTER(NwOptions), _vp]),
+    "msa_nw_plan_execute": (ctypes.c_int, [_vp, _vp, _vp]),
+    "msa_nw_plan_info_get": (ctypes.c_int, [_vp, ctypes.POINTER(PlanInfo)]),
+    "msa_nw_plan_destroy": (ctypes.c_int, [_vp]),
+    "msa_nw_align": (ctypes.c_int, [_vp, _vp, _vp, _i32, _vp, _i64, _vp, _i32, ctypes.POINTER(NwOptions), _vp]),
+    "msa_nw_plain": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _vp, _i64, _vp, _i32, _vp, _vp]),
+}
+
+
+class MsaError(RuntimeError):
+    def __init__(self, status, detail):
+        self.status = status
+        super().__init__("%s (status %d)" % (detail, status))
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("msa_amd native library not built (%s missing): run "
+                               "`python -m msa_amd.build`" % LIB_PATH)
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(status):
+    if status != MSA_OK:
+        detail = lib().msa_last_error().decode(errors="replace") or lib().msa_error_string(status).decode()
+        raise MsaError(status, detail)
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    check(lib().msa_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def device_props(device=0):
+    p = DeviceProps()
+    check(lib().msa_device_props_get(device, ctypes.byref(p)))
+    return p
+
+
+def table(name):
+    """(int32[25,25] table, penalty) of a named table."""
+    t = np.zeros(625, dtype=np.int32)
+    pen = _i32(0)
+    check(lib().msa_table_get(name.encode(), t.ctypes.data, ctypes.byref(pen)))
+    return t.reshape(25, 25), pen.value
+
+
+def table_names():
+    return [lib().msa_table_name(i).decode() for i in range(lib().msa_table_count())]
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None and a.size else None
+
+
+class DeviceBuffer:
+    """A device allocation owned by Python (freed on close or collection)."""
+
+    def __init__(self, nbytes):
+        p = _vp()
+        check(lib().msa_malloc(ctypes.byref(p), max(int(nbytes), 1)))
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+
+    def upload(self, array):
+        array = np.ascontiguousarray(array)
+        assert array.nbytes <= self.nbytes
+        check(lib().msa_memcpy(self.ptr, array.ctypes.data, array.nbytes))
+
+    def download(self, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        if count:
+            check(lib().msa_memcpy(out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def close(self):
+        if self.ptr:
+            lib().msa_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def options(kernel=KERNEL_AUTO, use_graph=False, lanes=0, rows=0, long_threshold=0):
+    return NwOptions(kernel, 1 if use_graph else 0, lanes, rows, long_threshold, None)
+
+
+class Plan:
+    """An msa_nw_plan: resident inputs and work lists for one pair set."""
+
+    def __init__(self, blocks, offsets, sizes, table625, penalty, pairs=None, opts=None, stream=None):
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint16)
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+        table625 = np.ascontiguousarray(np.asarray(table625).reshape(625), dtype=np.int32)
+        n = len(sizes)
+        if pairs is not None:
+            pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+            npairs = len(pairs)
+        else:
+            npairs = n * (n - 1) // 2
+        self.npairs = npairs
+        self.opts = opts or options()
+        h = _vp()
+        check(lib().msa_nw_plan_create(ctypes.byref(h), _ptr(blocks), _ptr(offsets), _ptr(sizes), n,
+                                       _ptr(pairs), npairs, table625.ctypes.data, int(penalty),
+                                       ctypes.byref(self.opts), stream))
+        self.handle = h.value
+
+    def info(self):
+        i = PlanInfo()
+        check(lib().msa_nw_plan_info_get(self.handle, ctypes.byref(i)))
+        return i
+
+    def execute(self, d_out, stream=None):
+        check(lib().msa_nw_plan_execute(self.handle, d_out, stream))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().msa_nw_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def align(blocks, offsets, sizes, table625, penalty, pairs=None, opts=None):
+    """msa_nw_align: host in, host out, int32 scores in pair order."""
+    blocks = np.ascontiguousarray(blocks, dtype=np.uint16)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+    table625 = np.ascontiguousarray(np.asarray(table625).reshape(625), dtype=np.int32)
+    n = len(sizes)
+    if pairs is not None:
+        pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        npairs = len(pairs)
+    else:
+        npairs = n * (n - 1) // 2
+    out = np.zeros(npairs, dtype=np.int32)
+    o = opts or options()
+    check(lib().msa_nw_align(_ptr(blocks), _ptr(offsets), _ptr(sizes), n, _ptr(pairs), npairs,
+                             table625.ctypes.data, int(penalty), ctypes.byref(o), _ptr(out)))
+    return out

@@ -1,0 +1,64 @@
+"""Builds the native parts of msa_amd in-tree.
+
+  libmsa_hip.so   the C-ABI library: HIP kernels for gfx950 plus their host
+                  planner and runtime layer (hipcc)
+  _museqa*.so     the Python binding of the C++ host layer (museqa::pairwise
+                  surface), linked against libmsa_hip.so
+
+Every HIP translation unit is compiled with packed FP32 instructions turned
+off (-Xclang -target-feature -Xclang -packed-fp32-ops); hipcc's host pass
+warns that it does not know that feature, which is harmless.
+"""
+import os
+import subprocess
+import sys
+import sysconfig
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("MSA_OFFLOAD_ARCH", "gfx950")
+
+HIP_SOURCES = ["runtime.hip", "nw_plain.hip", "nw_batched.hip", "nw_plan.hip"]
+HIP_FLAGS = ["-std=c++17", "-O3", "-fPIC", "--offload-arch=" + ARCH,
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wall", "-Wno-unused-command-line-argument"]
+
+
+def _run(cmd, cwd=None):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=cwd)
+
+
+def build_hip(jobs=4):
+    objdir = os.path.join(PKG, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    procs = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        objs.append(obj)
+        cmd = [HIPCC] + HIP_FLAGS + ["-I", INCLUDE, "-I", CSRC, "-c", os.path.join(CSRC, src), "-o", obj]
+        print("+", " ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd))
+        if len(procs) >= jobs:
+            if procs.pop(0).wait() != 0:
+                raise RuntimeError("hipcc failed")
+    for p in procs:
+        if p.wait() != 0:
+            raise RuntimeError("hipcc failed")
+    out = os.path.join(PKG, "libmsa_hip.so")
+    tmp = out + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
+    os.replace(tmp, out)
+    return out
+
+
+def build_all():
+    build_hip()
+
+
+if __name__ == "__main__":
+    build_all()

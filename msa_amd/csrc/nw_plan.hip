@@ -1,0 +1,525 @@
+// Host planner and the Needleman-Wunsch entry points of the C ABI.
+//
+// The planner replaces the reference's host batching (src/pairwise/needleman/
+// impl/hybrid.cu:262-429: required_memory, needed_cache, load_input,
+// make_input, align).  Where the reference cuts the pair list into batches
+// that fit free device memory and launches one thread block per pair, the
+// plan keeps the whole sequence store and every work list resident in HBM,
+// sorts pairs by shape and issues one persistent launch per length bin,
+// optionally captured in a hipGraph.
+//
+// Routing (every score keeps the reference's sequential semantics):
+//   - batched kernel: pairs whose column sequence (reference orientation,
+//     sequential.cpp:87-91) has no padding code before its last residue and
+//     whose row sequence has at least one row; their score is the global
+//     alignment of the two sequences cut at their first padding code;
+//   - plain kernel: every other pair (internal padding codes, empty sides),
+//     and every pair when MSA_KERNEL_PLAIN is asked for.
+#include "msa_internal.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <numeric>
+#include <vector>
+
+namespace msa {
+
+int plain_launch(const uint16_t* d_blocks, const SeqInfo* d_info, const int32_t* d_pairs,
+                 const int32_t* d_out_index, int64_t npairs, const Sxx& sxx, int32_t penalty,
+                 int32_t* d_out, int32_t* d_lines, int32_t line_stride, int grid, hipStream_t stream);
+int plain_grid(int64_t npairs, int compute_units);
+int seq_info_launch(const uint16_t* d_blocks, const int64_t* d_offset, const int32_t* d_size, int32_t nseq,
+                    SeqInfo* d_info, hipStream_t stream);
+bool batched_supported(int lanes, int rows);
+int batched_threads();
+size_t batched_lds_bytes(int lanes, int rows, int strips);
+int batched_groups_per_block(int lanes);
+int batched_occupancy(int lanes, int rows, size_t lds, int* blocks_per_cu);
+int batched_launch(int lanes, int rows, int grid, size_t lds, const uint16_t* d_blocks, const SeqInfo* d_info,
+                   const int4* d_items, int32_t nitems, const int2* d_targets, int32_t* d_queue, int32_t* d_out,
+                   int32_t* d_lines, int32_t line_stride, const Sxx& sxx, int32_t penalty, hipStream_t stream);
+
+namespace {
+
+// Host view of one sequence (the same facts as SeqInfo).
+struct HostSeq {
+    int64_t offset;
+    int32_t blocks;
+    int32_t first_end;   // rows the reference runs for it as the row sequence
+    int32_t last_real;   // columns up to its last non-padding code
+};
+
+int scan_sequences(const uint16_t* blocks, const int64_t* offset, const int32_t* size, int32_t nseq,
+                   std::vector<HostSeq>& out, int64_t& total_blocks)
+{
+    out.resize(nseq);
+    total_blocks = 0;
+    for (int32_t s = 0; s < nseq; ++s) {
+        if (size[s] < 0 || offset[s] < 0)
+            return fail(MSA_ERR_INVALID_ARGUMENT, "sequence %d has a negative offset or size", s);
+        total_blocks = std::max<int64_t>(total_blocks, offset[s] + size[s]);
+        HostSeq h{offset[s], size[s], 3 * size[s], 0};
+        const uint16_t* seq = blocks + offset[s];
+        for (int32_t pos = 0; pos < 3 * size[s]; ++pos) {
+            const int code = (seq[pos / 3] >> (1 + 5 * (pos % 3))) & 0x1F;
+            if (code > kEndCode)
+                return fail(MSA_ERR_GAP_CODE, "sequence %d holds code %d at position %d (the reference "
+                            "scores only codes 0..24; 0x19 is the gap code)", s, code, pos);
+            if (code == kEndCode) h.first_end = std::min(h.first_end, pos);
+            else h.last_real = pos + 1;
+        }
+        out[s] = h;
+    }
+    if (total_blocks >= (int64_t(1) << 31))
+        return fail(MSA_ERR_TOO_LARGE, "sequence store holds %lld blocks; at most 2^31-1 are supported",
+                    (long long)total_blocks);
+    return MSA_OK;
+}
+
+int make_sxx(const int32_t* table625, int32_t penalty, Sxx& sxx, bool& symmetric)
+{
+    std::memset(&sxx, 0, sizeof(sxx));
+    symmetric = true;
+    for (int a = 0; a < 25; ++a)
+        for (int b = 0; b < 25; ++b) {
+            const int64_t v = (int64_t)table625[a * 25 + b] + 2 * (int64_t)penalty;
+            if (v < 0 || v > 255)
+                return fail(MSA_ERR_TABLE, "table[%d][%d] + 2*penalty = %lld is outside 0..255, which the "
+                            "byte profile of the kernels holds", a, b, (long long)v);
+            sxx.v[a * 25 + b] = (uint8_t)v;
+            if (table625[a * 25 + b] != table625[b * 25 + a]) symmetric = false;
+        }
+    return MSA_OK;
+}
+
+struct DeviceState {
+    int compute_units = 0;
+    size_t lds_per_block = 0;
+};
+
+int device_state(DeviceState& st)
+{
+    int dev = 0;
+    MSA_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    MSA_HIP(hipGetDeviceProperties(&p, dev));
+    st.compute_units = p.multiProcessorCount;
+    st.lds_per_block = p.sharedMemPerBlock;
+    return MSA_OK;
+}
+
+struct Bin {
+    int lanes = 16, rows = 16, strips_cap = 1, grid = 1;
+    size_t lds = 0;
+    int32_t nitems = 0;
+    int32_t line_stride = 1;
+    int4* d_items = nullptr;
+    int2* d_targets = nullptr;
+    int32_t* d_lines = nullptr;
+    int32_t* d_queue = nullptr;
+};
+
+}  // namespace
+}  // namespace msa
+
+struct msa_nw_plan_opaque {
+    int device = 0;
+    int32_t penalty = 0;
+    msa::Sxx sxx;
+    msa_nw_plan_info info{};
+    uint16_t* d_blocks = nullptr;
+    msa::SeqInfo* d_info = nullptr;
+    // plain part
+    int64_t n_plain = 0;
+    int32_t* d_plain_pairs = nullptr;
+    int32_t* d_plain_index = nullptr;
+    int32_t* d_plain_lines = nullptr;
+    int32_t plain_line_stride = 1;
+    int plain_grid = 0;
+    // batched part
+    std::vector<msa::Bin> bins;
+    // graph
+    bool use_graph = false;
+    hipStream_t capture_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int32_t* graph_out = nullptr;
+    std::vector<void*> allocations;
+
+    ~msa_nw_plan_opaque()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (capture_stream) (void)hipStreamDestroy(capture_stream);
+        for (void* p : allocations) (void)hipFree(p);
+    }
+
+    template <typename T>
+    int alloc(T*& ptr, size_t count)
+    {
+        void* p = nullptr;
+        MSA_HIP(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+        allocations.push_back(p);
+        info.device_bytes += (int64_t)(std::max<size_t>(count, 1) * sizeof(T));
+        ptr = static_cast<T*>(p);
+        return MSA_OK;
+    }
+
+    template <typename T>
+    int upload(T*& ptr, const T* host, size_t count, hipStream_t stream)
+    {
+        MSA_TRY(alloc(ptr, count));
+        // The host arrays are temporaries of the planner: copy synchronously.
+        if (count) {
+            MSA_HIP(hipMemcpyAsync(ptr, host, count * sizeof(T), hipMemcpyHostToDevice, stream));
+            MSA_HIP(hipStreamSynchronize(stream));
+        }
+        return MSA_OK;
+    }
+
+    int launch_all(int32_t* d_out, hipStream_t stream)
+    {
+        for (const auto& b : bins) {
+            if (b.nitems == 0) continue;
+            MSA_TRY(msa::batched_launch(b.lanes, b.rows, b.grid, b.lds, d_blocks, d_info, b.d_items, b.nitems,
+                                        b.d_targets, b.d_queue, d_out, b.d_lines, b.line_stride, sxx, penalty,
+                                        stream));
+        }
+        if (n_plain)
+            MSA_TRY(msa::plain_launch(d_blocks, d_info, d_plain_pairs, d_plain_index, n_plain, sxx, penalty, d_out,
+                                      d_plain_lines, plain_line_stride, plain_grid, stream));
+        return MSA_OK;
+    }
+};
+
+namespace msa {
+namespace {
+
+int build_plan(msa_nw_plan_opaque& plan, const uint16_t* blocks, const int64_t* offset, const int32_t* size,
+               int32_t nseq, const int32_t* pairs, int64_t npairs, const int32_t* table625, int32_t penalty,
+               const msa_nw_options& opt, hipStream_t stream)
+{
+    if (nseq < 0 || npairs < 0) return fail(MSA_ERR_INVALID_ARGUMENT, "negative sizes");
+    if (nseq > 0 && (!offset || !size)) return fail(MSA_ERR_INVALID_ARGUMENT, "null sequence views");
+    if (!table625) return fail(MSA_ERR_INVALID_ARGUMENT, "null table");
+    if (!pairs && npairs != (int64_t)nseq * (nseq - 1) / 2)
+        return fail(MSA_ERR_INVALID_ARGUMENT, "all-pairs mode needs npairs = nseq*(nseq-1)/2 (%lld), got %lld",
+                    (long long)nseq * (nseq - 1) / 2, (long long)npairs);
+    if (npairs >= (int64_t(1) << 31))
+        return fail(MSA_ERR_TOO_LARGE, "%lld pairs; at most 2^31-1 per plan", (long long)npairs);
+    const int kernel = opt.kernel;
+    if (kernel < MSA_KERNEL_AUTO || kernel > MSA_KERNEL_LONG)
+        return fail(MSA_ERR_INVALID_ARGUMENT, "unknown kernel %d", kernel);
+
+    std::vector<HostSeq> seqs;
+    int64_t total_blocks = 0;
+    MSA_TRY(scan_sequences(blocks, offset, size, nseq, seqs, total_blocks));
+    bool symmetric = true;
+    MSA_TRY(make_sxx(table625, penalty, plan.sxx, symmetric));
+    plan.penalty = penalty;
+
+    DeviceState dev;
+    MSA_TRY(device_state(dev));
+    MSA_HIP(hipGetDevice(&plan.device));
+
+    const int lanes = opt.lanes_per_pair ? opt.lanes_per_pair : 16;
+    const int rows = opt.rows_per_lane ? opt.rows_per_lane : 16;
+    if (!batched_supported(lanes, rows))
+        return fail(MSA_ERR_INVALID_ARGUMENT, "no batched kernel for %d lanes x %d rows per lane", lanes, rows);
+    const int strip = lanes * rows;
+    const int max_strips = (int)std::min<size_t>(64, dev.lds_per_block / batched_lds_bytes(lanes, rows, 1));
+
+    // Classify pairs.
+    std::vector<int32_t> plain_pairs, plain_index;
+    struct Fast { int32_t query, target, index; };
+    std::vector<Fast> fast;
+    fast.reserve(kernel == MSA_KERNEL_PLAIN ? 0 : (size_t)npairs);
+    int64_t cells = 0;
+    int64_t c = 0;
+    auto visit = [&](int32_t first, int32_t second, int64_t idx) -> int {
+        if (first < 0 || first >= nseq || second < 0 || second >= nseq)
+            return fail(MSA_ERR_INVALID_ARGUMENT, "pair %lld names sequence outside 0..%d", (long long)idx, nseq - 1);
+        const int32_t rowref = seqs[first].blocks > seqs[second].blocks ? first : second;
+        const int32_t colref = rowref == first ? second : first;
+        const HostSeq& r = seqs[rowref];
+        const HostSeq& col = seqs[colref];
+        cells += (int64_t)r.first_end * col.last_real;
+        const bool clean = col.first_end == col.last_real && r.first_end > 0 && col.last_real > 0;
+        const bool fits = std::min(r.first_end, col.last_real) <= (int64_t)max_strips * strip;
+        if (kernel == MSA_KERNEL_PLAIN || !clean || !fits) {
+            plain_pairs.push_back(first);
+            plain_pairs.push_back(second);
+            plain_index.push_back((int32_t)idx);
+            return MSA_OK;
+        }
+        const bool row_query = r.first_end < col.last_real || (r.first_end == col.last_real && rowref < colref);
+        const int32_t q = row_query ? rowref : colref;
+        const int32_t t = row_query ? colref : rowref;
+        const int32_t transpose = (!symmetric && !row_query) ? 1 : 0;
+        fast.push_back({q * 2 + transpose, t, (int32_t)idx});
+        return MSA_OK;
+    };
+    if (pairs) {
+        for (c = 0; c < npairs; ++c) MSA_TRY(visit(pairs[2 * c], pairs[2 * c + 1], c));
+    } else {
+        for (int32_t i = 1; i < nseq; ++i)
+            for (int32_t j = 0; j < i; ++j, ++c) MSA_TRY(visit(i, j, c));
+    }
+
+    plan.info.pairs = npairs;
+    plan.info.cells = cells;
+    plan.info.plain_pairs = (int64_t)plain_index.size();
+    plan.info.batched_pairs = (int64_t)fast.size();
+
+    // Upload the sequence store; per-sequence facts come from the device scan
+    // so that msa_nw_plain and the plan agree on them.
+    std::vector<SeqInfo> hinfo(nseq);
+    for (int32_t s = 0; s < nseq; ++s)
+        hinfo[s] = make_int4((int)seqs[s].offset, seqs[s].blocks, seqs[s].first_end, seqs[s].last_real);
+    MSA_TRY(plan.upload(plan.d_blocks, blocks, (size_t)total_blocks, stream));
+    MSA_TRY(plan.upload(plan.d_info, hinfo.data(), hinfo.size(), stream));
+
+    // Plain part.
+    if (!plain_index.empty()) {
+        plan.n_plain = (int64_t)plain_index.size();
+        MSA_TRY(plan.upload(plan.d_plain_pairs, plain_pairs.data(), plain_pairs.size(), stream));
+        MSA_TRY(plan.upload(plan.d_plain_index, plain_index.data(), plain_index.size(), stream));
+        int32_t max_cols = 1;
+        for (size_t k = 0; k < plain_index.size(); ++k)
+            max_cols = std::max({max_cols, seqs[plain_pairs[2 * k]].last_real, seqs[plain_pairs[2 * k + 1]].last_real});
+        plan.plain_grid = plain_grid(plan.n_plain, dev.compute_units);
+        plan.plain_line_stride = (max_cols + 63) & ~63;
+        MSA_TRY(plan.alloc(plan.d_plain_lines, (size_t)plan.plain_grid * plan.plain_line_stride));
+    }
+
+    // Batched part: group by (query, transpose), sort targets by length,
+    // cut into items, bin items by strip count.
+    if (!fast.empty()) {
+        std::sort(fast.begin(), fast.end(), [&](const Fast& x, const Fast& y) {
+            if (x.query != y.query) return x.query < y.query;
+            const int32_t lx = seqs[x.target].first_end, ly = seqs[y.target].first_end;
+            if (lx != ly) return lx > ly;
+            return x.index < y.index;
+        });
+        const int groups = batched_groups_per_block(lanes);
+        const int per_item = groups * 4;
+        struct Item { int4 v; int64_t cost; int strips; };
+        std::vector<Item> items;
+        std::vector<int2> targets(fast.size());
+        for (size_t k = 0; k < fast.size(); ++k) targets[k] = make_int2(fast[k].target, fast[k].index);
+        size_t k = 0;
+        while (k < fast.size()) {
+            size_t e = k;
+            while (e < fast.size() && fast[e].query == fast[k].query) ++e;
+            const int32_t q = fast[k].query >> 1;
+            const int32_t flags = fast[k].query & 1;
+            const int lq = seqs[q].first_end;
+            const int strips = (lq + strip - 1) / strip;
+            for (size_t s = k; s < e; s += per_item) {
+                const int32_t cnt = (int32_t)std::min<size_t>(per_item, e - s);
+                int64_t cost = 0;
+                for (size_t u = s; u < s + cnt; ++u) cost += (int64_t)strips * strip * (seqs[fast[u].target].first_end + lanes - 1);
+                items.push_back({make_int4(q, (int)s, cnt, flags), cost, strips});
+            }
+            k = e;
+        }
+        // Bins by strip count: 1, 2, 3-4, 5-8, ... so that the LDS size of
+        // each launch fits its queries and short queries keep occupancy.
+        std::vector<std::vector<Item>> binned;
+        std::vector<int> caps;
+        for (const auto& it : items) {
+            int cap = 1, b = 0;
+            while (cap < it.strips) { cap *= 2; ++b; }
+            if ((int)binned.size() <= b) { binned.resize(b + 1); caps.resize(b + 1, 0); }
+            binned[b].push_back(it);
+            caps[b] = std::max(caps[b], it.strips);
+        }
+        int2* d_targets = nullptr;
+        MSA_TRY(plan.upload(d_targets, targets.data(), targets.size(), stream));
+        for (size_t b = 0; b < binned.size(); ++b) {
+            if (binned[b].empty()) continue;
+            auto& list = binned[b];
+            std::stable_sort(list.begin(), list.end(), [](const Item& x, const Item& y) { return x.cost > y.cost; });
+            Bin bin;
+            bin.lanes = lanes;
+            bin.rows = rows;
+            bin.strips_cap = caps[b];
+            bin.lds = batched_lds_bytes(lanes, rows, bin.strips_cap);
+            bin.nitems = (int32_t)list.size();
+            int per_cu = 0;
+            MSA_TRY(batched_occupancy(lanes, rows, bin.lds, &per_cu));
+            if (per_cu < 1) return fail(MSA_ERR_TOO_LARGE, "batched bin with %d strips does not fit a CU", bin.strips_cap);
+            bin.grid = (int)std::min<int64_t>((int64_t)per_cu * dev.compute_units, bin.nitems);
+            int32_t max_t = 1;
+            std::vector<int4> hitems(list.size());
+            for (size_t u = 0; u < list.size(); ++u) {
+                hitems[u] = list[u].v;
+                for (int32_t v = 0; v < list[u].v.z; ++v)
+                    max_t = std::max(max_t, seqs[targets[list[u].v.y + v].x].first_end);
+            }
+            bin.line_stride = bin.strips_cap > 1 ? ((max_t + 63) & ~63) : 1;
+            MSA_TRY(plan.upload(bin.d_items, hitems.data(), hitems.size(), stream));
+            bin.d_targets = d_targets;
+            MSA_TRY(plan.alloc(bin.d_lines, (size_t)bin.grid * batched_groups_per_block(lanes) * bin.line_stride));
+            MSA_TRY(plan.alloc(bin.d_queue, 1));
+            plan.bins.push_back(bin);
+        }
+        plan.info.bins = (int32_t)plan.bins.size();
+    }
+    plan.info.launches = (int32_t)(2 * plan.bins.size() + (plan.n_plain ? 1 : 0));
+    plan.use_graph = opt.use_graph != 0;
+    MSA_HIP(hipStreamSynchronize(stream));
+    return MSA_OK;
+}
+
+}  // namespace
+}  // namespace msa
+
+using msa::fail;
+
+extern "C" {
+
+int msa_nw_plan_create(msa_nw_plan* plan, const uint16_t* blocks, const int64_t* offset, const int32_t* size,
+                       int32_t nseq, const int32_t* pairs, int64_t npairs, const int32_t* table625, int32_t penalty,
+                       const msa_nw_options* options, msa_stream stream)
+{
+    if (!plan) return fail(MSA_ERR_INVALID_ARGUMENT, "plan is null");
+    *plan = nullptr;
+    if (nseq > 0 && !blocks) return fail(MSA_ERR_INVALID_ARGUMENT, "blocks is null");
+    msa_nw_options opt{};
+    if (options) opt = *options;
+    auto* p = new (std::nothrow) msa_nw_plan_opaque();
+    if (!p) return fail(MSA_ERR_OUT_OF_MEMORY, "host allocation failed");
+    int rc = MSA_OK;
+    try {
+        rc = msa::build_plan(*p, blocks, offset, size, nseq, pairs, npairs, table625, penalty, opt,
+                             (hipStream_t)stream);
+    } catch (const std::bad_alloc&) {
+        rc = fail(MSA_ERR_OUT_OF_MEMORY, "host allocation failed while planning");
+    }
+    if (rc != MSA_OK) {
+        delete p;
+        return rc;
+    }
+    *plan = p;
+    return MSA_OK;
+}
+
+int msa_nw_plan_execute(msa_nw_plan plan, int32_t* d_out, msa_stream stream)
+{
+    if (!plan || (!d_out && plan->info.pairs)) return fail(MSA_ERR_INVALID_ARGUMENT, "null plan or output");
+    if (!plan->info.pairs) return MSA_OK;
+    const hipStream_t s = (hipStream_t)stream;
+    if (!plan->use_graph) return plan->launch_all(d_out, s);
+    if (plan->graph_out != d_out || !plan->exec) {
+        if (plan->exec) { (void)hipGraphExecDestroy(plan->exec); plan->exec = nullptr; }
+        if (plan->graph) { (void)hipGraphDestroy(plan->graph); plan->graph = nullptr; }
+        if (!plan->capture_stream) MSA_HIP(hipStreamCreateWithFlags(&plan->capture_stream, hipStreamNonBlocking));
+        MSA_HIP(hipStreamBeginCapture(plan->capture_stream, hipStreamCaptureModeThreadLocal));
+        const int rc = plan->launch_all(d_out, plan->capture_stream);
+        hipGraph_t g = nullptr;
+        const hipError_t end = hipStreamEndCapture(plan->capture_stream, &g);
+        if (rc != MSA_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
+        MSA_HIP(end);
+        plan->graph = g;
+        MSA_HIP(hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0));
+        plan->graph_out = d_out;
+    }
+    return msa::hip_status(hipGraphLaunch(plan->exec, s), "hipGraphLaunch");
+}
+
+int msa_nw_plan_info_get(msa_nw_plan plan, msa_nw_plan_info* info)
+{
+    if (!plan || !info) return fail(MSA_ERR_INVALID_ARGUMENT, "null plan or info");
+    *info = plan->info;
+    return MSA_OK;
+}
+
+int msa_nw_plan_destroy(msa_nw_plan plan)
+{
+    delete plan;
+    return MSA_OK;
+}
+
+int msa_nw_align(const uint16_t* blocks, const int64_t* offset, const int32_t* size, int32_t nseq,
+                 const int32_t* pairs, int64_t npairs, const int32_t* table625, int32_t penalty,
+                 const msa_nw_options* options, int32_t* out)
+{
+    if (npairs > 0 && !out) return fail(MSA_ERR_INVALID_ARGUMENT, "out is null");
+    msa_nw_plan plan = nullptr;
+    MSA_TRY(msa_nw_plan_create(&plan, blocks, offset, size, nseq, pairs, npairs, table625, penalty, options, nullptr));
+    int32_t* d_out = nullptr;
+    int rc = MSA_OK;
+    if (npairs > 0) {
+        rc = msa::hip_status(hipMalloc(&d_out, (size_t)npairs * sizeof(int32_t)), "hipMalloc");
+        if (rc == MSA_OK) rc = msa_nw_plan_execute(plan, d_out, nullptr);
+        if (rc == MSA_OK)
+            rc = msa::hip_status(hipMemcpy(out, d_out, (size_t)npairs * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy");
+        if (d_out) (void)hipFree(d_out);
+    }
+    msa_nw_plan_destroy(plan);
+    return rc;
+}
+
+}  // extern "C"
+
+// ---- msa_nw_plain on caller-owned device memory -----------------------------
+
+namespace {
+
+struct PlainWorkspace {
+    std::mutex lock;
+    msa::SeqInfo* info = nullptr;
+    size_t info_cap = 0;
+    int32_t* lines = nullptr;
+    size_t lines_cap = 0;
+};
+
+PlainWorkspace g_workspace[64];
+
+template <typename T>
+int grow(T*& ptr, size_t& cap, size_t need)
+{
+    if (need <= cap) return MSA_OK;
+    if (ptr) MSA_HIP(hipFree(ptr));
+    ptr = nullptr;
+    cap = 0;
+    void* p = nullptr;
+    MSA_HIP(hipMalloc(&p, need * sizeof(T)));
+    ptr = static_cast<T*>(p);
+    cap = need;
+    return MSA_OK;
+}
+
+}  // namespace
+
+extern "C" int msa_nw_plain(const uint16_t* d_blocks, const int64_t* d_offset, const int32_t* d_size, int32_t nseq,
+                            int32_t max_blocks, const int32_t* d_pairs, int64_t npairs, const int32_t* table625,
+                            int32_t penalty, int32_t* d_out, msa_stream stream)
+{
+    if (npairs == 0) return MSA_OK;
+    if (npairs < 0 || nseq < 1 || max_blocks < 0 || !d_blocks || !d_offset || !d_size || !d_out || !table625)
+        return fail(MSA_ERR_INVALID_ARGUMENT, "invalid arguments to msa_nw_plain");
+    if (!d_pairs && npairs != (int64_t)nseq * (nseq - 1) / 2)
+        return fail(MSA_ERR_INVALID_ARGUMENT, "all-pairs mode needs npairs = nseq*(nseq-1)/2");
+    msa::Sxx sxx;
+    bool symmetric = true;
+    MSA_TRY(msa::make_sxx(table625, penalty, sxx, symmetric));
+    int dev = 0;
+    MSA_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(MSA_ERR_INVALID_ARGUMENT, "device %d out of range", dev);
+    msa::DeviceState st;
+    MSA_TRY(msa::device_state(st));
+    PlainWorkspace& ws = g_workspace[dev];
+    std::lock_guard<std::mutex> guard(ws.lock);
+    const int grid = msa::plain_grid(npairs, st.compute_units);
+    const int32_t stride = (int32_t)(((int64_t)3 * max_blocks + 63) & ~int64_t(63));
+    MSA_TRY(grow(ws.info, ws.info_cap, (size_t)nseq));
+    MSA_TRY(grow(ws.lines, ws.lines_cap, (size_t)grid * std::max(stride, 1)));
+    const hipStream_t s = (hipStream_t)stream;
+    MSA_TRY(msa::seq_info_launch(d_blocks, d_offset, d_size, nseq, ws.info, s));
+    return msa::plain_launch(d_blocks, ws.info, d_pairs, nullptr, npairs, sxx, penalty, d_out, ws.lines,
+                             std::max(stride, 1), grid, s);
+}

@@ -1,0 +1,209 @@
+"""ORACLE -- test infrastructure, not part of the product.
+
+Python restatements of the reference's host-side behaviour on the pairwise
+path.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+import this module, and only as the checker.
+
+Contents and what each follows:
+  encode_char / encode / decode ......... src/encoder.cpp:27-118 (letter map,
+                                          3 codes per uint16 block at shifts
+                                          1/6/11, bit 0 = block holds padding)
+  parse_fasta ........................... src/io/loader/parser/fasta.cpp:26-70
+  needleman ............................. src/python/algorithm/pairwise/needleman.py:19-63
+                                          (pure-Python loops: small inputs only)
+  pair_order ............................ src/pairwise/pairwise.cpp:73-85
+  textbook_nw ........................... independent textbook global alignment
+                                          with a linear gap (full matrix, int64)
+  CpuOracle ............................. ctypes wrapper of nw_oracle.c, the C
+                                          restatement of sequential.cpp:29-95
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+END = 0x18
+GAP = 0x19
+SHIFT = (1, 6, 11)
+
+# src/encoder.cpp:31-34, indexed by upper - 'A'
+_ENCODE = (0x00, 0x14, 0x01, 0x06, 0x08, 0x0E, 0x03, 0x09, 0x0A, 0x15, 0x0C, 0x0B, 0x0D,
+           0x05, 0x17, 0x0F, 0x07, 0x04, 0x10, 0x02, 0x17, 0x13, 0x11, 0x17, 0x12, 0x16)
+# src/encoder.cpp:40-43
+_DECODE = "ACTGRNDQEHILKMFPSWYVBJZX*-"
+
+
+def encode_char(ch):
+    """encoder::encode(char), src/encoder.cpp:50-61."""
+    upper = ch.upper() if "a" <= ch <= "z" else ch
+    if "A" <= upper <= "Z":
+        return _ENCODE[ord(upper) - ord("A")]
+    if ch == "-":
+        return GAP
+    return END
+
+
+def encode(text):
+    """encoder::encode(ptr, size), src/encoder.cpp:76-98 -> list of uint16 blocks."""
+    size = len(text)
+    blocks = []
+    n = 0
+    while n < size:
+        block = 1 if n + 3 > size else 0
+        for k in range(3):
+            code = encode_char(text[n]) if n < size else END
+            block |= code << SHIFT[k]
+            n += 1
+        blocks.append(block & 0xFFFF)
+    return blocks
+
+
+def access(blocks, offset):
+    """encoder::access, src/encoder.hpp:54-70."""
+    return (blocks[offset // 3] >> SHIFT[offset % 3]) & 0x1F
+
+
+def decode(blocks):
+    """encoder::decode(buffer), src/encoder.cpp:105-118 (padded string)."""
+    return "".join(_DECODE[access(blocks, i)] for i in range(3 * len(blocks)))
+
+
+def parse_fasta(path):
+    """io::parser::fasta, src/io/loader/parser/fasta.cpp:26-70.
+
+    A record starts at a '>' line and takes the following lines up to the
+    next '>' or the first empty line.  Returns [(description, sequence)].
+    """
+    with open(path, "r", newline="") as f:
+        text = f.read()
+    lines = text.split("\n")
+    has_final_newline = text.endswith("\n")
+    if has_final_newline:
+        lines = lines[:-1]
+    out = []
+    k = 0
+    while k < len(lines):
+        while k < len(lines) and (len(lines[k]) < 1 or lines[k][0] != ">"):
+            k += 1
+        if k >= len(lines):
+            break
+        description = lines[k][1:]
+        k += 1
+        contents = []
+        while k < len(lines) and not lines[k].startswith(">") and len(lines[k]) > 0:
+            contents.append(lines[k])
+            k += 1
+        if k < len(lines) and len(lines[k]) == 0:
+            k += 1
+        out.append((description, "".join(contents)))
+    return out
+
+
+def pair_order(n):
+    """algorithm::generate, src/pairwise/pairwise.cpp:73-85."""
+    return [(i, j) for i in range(1, n) for j in range(i)]
+
+
+def load_tables():
+    with open(os.path.join(HERE, "tables.json")) as f:
+        data = json.load(f)
+    return {t["name"]: (np.array(t["matrix"], dtype=np.int64), t["penalty"]) for t in data["tables"]}
+
+
+def needleman(seqs, table, penalty):
+    """needleman.py:19-63 over block lists; returns float scores in pair order.
+
+    `seqs` are block lists, `table` a 25x25 nested list or array, `penalty`
+    the gap penalty.  Padding is code 24, as Sequence.padding ('*') is.
+    """
+    def align(one, two):
+        if 3 * len(one) <= 3 * len(two):
+            one, two = two, one
+        n1, n2 = 3 * len(one), 3 * len(two)
+        line = [float(i * -penalty) for i in range(n2 + 1)]
+        for i in range(n1):
+            a = access(one, i)
+            if a == END:
+                break
+            done = line[0]
+            line[0] = float((i + 1) * -penalty)
+            for j in range(1, n2 + 1):
+                value = line[j - 1]
+                b = access(two, j - 1)
+                if b != END:
+                    insertd = value - penalty
+                    removed = line[j] - penalty
+                    matched = done + float(table[a][b])
+                    value = max(insertd, removed, matched)
+                done = line[j]
+                line[j] = value
+        return line[-1]
+
+    return [align(seqs[i], seqs[j]) for i, j in pair_order(len(seqs))]
+
+
+def textbook_nw(a, b, table, penalty):
+    """Textbook global alignment score, linear gap, full DP in int64.
+
+    H[i][0] = -i*p, H[0][j] = -j*p,
+    H[i][j] = max(H[i-1][j-1] + T[a_i][b_j], H[i-1][j] - p, H[i][j-1] - p).
+    Each row is computed with a prefix maximum:  H[i][j] = max_k<=j (t_k - (j-k)p).
+    """
+    a = np.asarray(a, dtype=np.int64)
+    b = np.asarray(b, dtype=np.int64)
+    table = np.asarray(table, dtype=np.int64)
+    p = int(penalty)
+    m = len(b)
+    jp = np.arange(m + 1, dtype=np.int64) * p
+    prev = -jp.copy()
+    for i in range(1, len(a) + 1):
+        t = np.empty(m + 1, dtype=np.int64)
+        t[0] = -i * p
+        if m:
+            t[1:] = np.maximum(prev[:-1] + table[a[i - 1], b], prev[1:] - p)
+        prev = np.maximum.accumulate(t + jp) - jp
+    return int(prev[m])
+
+
+class CpuOracle:
+    """ctypes wrapper of oracle/_build/liboracle.so (nw_oracle.c)."""
+
+    def __init__(self, path=None):
+        path = path or os.path.join(HERE, "_build", "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle library not built: run `make -C oracle`")
+        self.lib = ctypes.CDLL(path)
+        self.lib.oracle_align_range.argtypes = [
+            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+            ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_float,
+            ctypes.c_void_p, ctypes.c_int]
+        self.lib.oracle_align_range.restype = ctypes.c_int
+
+    def align(self, blocks, offsets, nblocks, table625, penalty, pairs=None,
+              first=0, count=None, workers=1):
+        """Scores of pairs [first, first+count) as float32 (reference order)."""
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint16)
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        nblocks = np.ascontiguousarray(nblocks, dtype=np.int64)
+        table = np.ascontiguousarray(np.asarray(table625, dtype=np.float32).reshape(625))
+        n = len(nblocks)
+        if pairs is not None:
+            pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+            total = len(pairs)
+        else:
+            total = n * (n - 1) // 2
+        if count is None:
+            count = total - first
+        out = np.zeros(count, dtype=np.float32)
+        if count == 0:
+            return out
+        rc = self.lib.oracle_align_range(
+            blocks.ctypes.data if len(blocks) else None, offsets.ctypes.data, nblocks.ctypes.data,
+            pairs.ctypes.data if pairs is not None else None, first, count,
+            table.ctypes.data, float(penalty), out.ctypes.data, int(workers))
+        if rc != 0:
+            raise RuntimeError("oracle failed")
+        return out

@@ -1,0 +1,48 @@
+/*
+ * ORACLE -- test infrastructure, not part of the product.
+ *
+ * Compiles the reference's own sequential Needleman-Wunsch translation unit
+ * (src/pairwise/needleman/impl/sequential.cpp, with src/encoder.cpp) from
+ * its sources under the reference tree, with the reference's production
+ * flags (-std=c++14 -O3 -DPRODUCTION, makefile:36-39,69-71), and exposes
+ * its file-local align_pair (sequential.cpp:29-68) through one C entry point.
+ * The build recipe is oracle/Makefile (target `ref`); the output goes to
+ * oracle/_ref/, which is never committed.  Tests use it to pin the
+ * restatement in nw_oracle.c; bench.py may time it as the CPU baseline.
+ *
+ * Nothing from the reference is copied here: the translation unit is pulled
+ * in by #include from the reference tree at build time.
+ */
+#include "pairwise/needleman/impl/sequential.cpp"
+
+#include <cstdint>
+#include <cstring>
+
+namespace
+{
+    museqa::sequence make_sequence(const uint16_t* blocks, size_t n)
+    {
+        auto buf = museqa::encoder::buffer::make(n);
+        for (size_t i = 0; i < n; ++i) buf[i] = blocks[i];
+        return museqa::sequence {buf};
+    }
+}
+
+extern "C" __attribute__((visibility("default")))
+float ref_align_pair(const uint16_t* one, size_t one_blocks, const uint16_t* two, size_t two_blocks,
+                     const float* table625, float penalty)
+{
+    using namespace museqa::pairwise;
+    static thread_local scoring_table::raw_type raw;
+    std::memcpy(&raw, table625, sizeof(raw));
+    const scoring_table table {museqa::pointer<scoring_table::raw_type>::weak(&raw), penalty};
+    return align_pair(make_sequence(one, one_blocks), make_sequence(two, two_blocks), table);
+}
+
+extern "C" __attribute__((visibility("default")))
+size_t ref_encode(const char* text, size_t size, uint16_t* out, size_t capacity)
+{
+    const auto encoded = museqa::encoder::encode(text, size);
+    for (size_t i = 0; i < encoded.size() && i < capacity; ++i) out[i] = encoded[i];
+    return encoded.size();
+}
