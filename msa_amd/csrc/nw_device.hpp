@@ -34,6 +34,21 @@ __device__ __forceinline__ int dpp(int old, int src)
     return __builtin_amdgcn_update_dpp(old, src, CTRL, ROW_MASK, 0xF, false);
 }
 
+// DPP move whose lanes without a source lane read 0.
+template <int CTRL>
+__device__ __forceinline__ int dpp_zero(int src)
+{
+    return __builtin_amdgcn_mov_dpp(src, CTRL, 0xF, 0xF, true);
+}
+
+// Words in LDS, addressed with 32-bit offsets.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(3))) uint8_t lds_u8;
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) u32x2 lds_u32x2;
+typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
+
 // Lane k of a G-lane group receives lane k-1's `src`; the first lane of each
 // group receives its own `first`.
 template <int G>

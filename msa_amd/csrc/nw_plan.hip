@@ -32,14 +32,14 @@ int plain_launch(const uint16_t* d_blocks, const SeqInfo* d_info, const int32_t*
 int plain_grid(int64_t npairs, int compute_units);
 int seq_info_launch(const uint16_t* d_blocks, const int64_t* d_offset, const int32_t* d_size, int32_t nseq,
                     SeqInfo* d_info, hipStream_t stream);
-bool batched_supported(int lanes, int rows);
 int batched_threads();
-size_t batched_lds_bytes(int lanes, int rows, int strips);
-int batched_groups_per_block(int lanes);
-int batched_occupancy(int lanes, int rows, size_t lds, int* blocks_per_cu);
-int batched_launch(int lanes, int rows, int grid, size_t lds, const uint16_t* d_blocks, const SeqInfo* d_info,
-                   const int4* d_items, int32_t nitems, const int2* d_targets, int32_t* d_queue, int32_t* d_out,
-                   int32_t* d_lines, int32_t line_stride, const Sxx& sxx, int32_t penalty, hipStream_t stream);
+size_t batched_lds_bytes();
+int batched_groups_per_block();
+int batched_strip_rows();
+int batched_occupancy(int* blocks_per_cu);
+int batched_launch(int grid, const uint16_t* d_blocks, const SeqInfo* d_info, const int4* d_items, int32_t nitems,
+                   const int2* d_targets, int32_t* d_queue, int32_t* d_out, int32_t* d_lines, int32_t line_stride,
+                   const Sxx& sxx, int32_t penalty, hipStream_t stream);
 
 namespace {
 
@@ -110,9 +110,8 @@ int device_state(DeviceState& st)
     return MSA_OK;
 }
 
-struct Bin {
-    int lanes = 16, rows = 16, strips_cap = 1, grid = 1;
-    size_t lds = 0;
+struct Batch {
+    int grid = 0;
     int32_t nitems = 0;
     int32_t line_stride = 1;
     int4* d_items = nullptr;
@@ -139,7 +138,7 @@ struct msa_nw_plan_opaque {
     int32_t plain_line_stride = 1;
     int plain_grid = 0;
     // batched part
-    std::vector<msa::Bin> bins;
+    msa::Batch batch;
     // graph
     bool use_graph = false;
     hipStream_t capture_stream = nullptr;
@@ -181,12 +180,9 @@ struct msa_nw_plan_opaque {
 
     int launch_all(int32_t* d_out, hipStream_t stream)
     {
-        for (const auto& b : bins) {
-            if (b.nitems == 0) continue;
-            MSA_TRY(msa::batched_launch(b.lanes, b.rows, b.grid, b.lds, d_blocks, d_info, b.d_items, b.nitems,
-                                        b.d_targets, b.d_queue, d_out, b.d_lines, b.line_stride, sxx, penalty,
-                                        stream));
-        }
+        if (batch.nitems)
+            MSA_TRY(msa::batched_launch(batch.grid, d_blocks, d_info, batch.d_items, batch.nitems, batch.d_targets,
+                                        batch.d_queue, d_out, batch.d_lines, batch.line_stride, sxx, penalty, stream));
         if (n_plain)
             MSA_TRY(msa::plain_launch(d_blocks, d_info, d_plain_pairs, d_plain_index, n_plain, sxx, penalty, d_out,
                                       d_plain_lines, plain_line_stride, plain_grid, stream));
@@ -224,13 +220,6 @@ int build_plan(msa_nw_plan_opaque& plan, const uint16_t* blocks, const int64_t* 
     MSA_TRY(device_state(dev));
     MSA_HIP(hipGetDevice(&plan.device));
 
-    const int lanes = opt.lanes_per_pair ? opt.lanes_per_pair : 16;
-    const int rows = opt.rows_per_lane ? opt.rows_per_lane : 16;
-    if (!batched_supported(lanes, rows))
-        return fail(MSA_ERR_INVALID_ARGUMENT, "no batched kernel for %d lanes x %d rows per lane", lanes, rows);
-    const int strip = lanes * rows;
-    const int max_strips = (int)std::min<size_t>(64, dev.lds_per_block / batched_lds_bytes(lanes, rows, 1));
-
     // Classify pairs.
     std::vector<int32_t> plain_pairs, plain_index;
     struct Fast { int32_t query, target, index; };
@@ -247,8 +236,7 @@ int build_plan(msa_nw_plan_opaque& plan, const uint16_t* blocks, const int64_t* 
         const HostSeq& col = seqs[colref];
         cells += (int64_t)r.first_end * col.last_real;
         const bool clean = col.first_end == col.last_real && r.first_end > 0 && col.last_real > 0;
-        const bool fits = std::min(r.first_end, col.last_real) <= (int64_t)max_strips * strip;
-        if (kernel == MSA_KERNEL_PLAIN || !clean || !fits) {
+        if (kernel == MSA_KERNEL_PLAIN || !clean) {
             plain_pairs.push_back(first);
             plain_pairs.push_back(second);
             plain_index.push_back((int32_t)idx);
@@ -294,8 +282,9 @@ int build_plan(msa_nw_plan_opaque& plan, const uint16_t* blocks, const int64_t* 
         MSA_TRY(plan.alloc(plan.d_plain_lines, (size_t)plan.plain_grid * plan.plain_line_stride));
     }
 
-    // Batched part: group by (query, transpose), sort targets by length,
-    // cut into items, bin items by strip count.
+    // Batched part: group by (query, transpose), sort each query's targets by
+    // length, cut them into items of up to kItemTargets, and queue the items
+    // longest first.
     if (!fast.empty()) {
         std::sort(fast.begin(), fast.end(), [&](const Fast& x, const Fast& y) {
             if (x.query != y.query) return x.query < y.query;
@@ -303,72 +292,52 @@ int build_plan(msa_nw_plan_opaque& plan, const uint16_t* blocks, const int64_t* 
             if (lx != ly) return lx > ly;
             return x.index < y.index;
         });
-        const int groups = batched_groups_per_block(lanes);
-        const int per_item = groups * 4;
-        struct Item { int4 v; int64_t cost; int strips; };
+        constexpr int kItemTargets = 16;
+        const int strip_rows = batched_strip_rows();
+        struct Item { int4 v; int64_t cost; };
         std::vector<Item> items;
         std::vector<int2> targets(fast.size());
-        for (size_t k = 0; k < fast.size(); ++k) targets[k] = make_int2(fast[k].target, fast[k].index);
+        int32_t max_t = 1;
+        for (size_t k = 0; k < fast.size(); ++k) {
+            targets[k] = make_int2(fast[k].target, fast[k].index);
+            max_t = std::max(max_t, seqs[fast[k].target].first_end);
+        }
         size_t k = 0;
         while (k < fast.size()) {
             size_t e = k;
             while (e < fast.size() && fast[e].query == fast[k].query) ++e;
             const int32_t q = fast[k].query >> 1;
             const int32_t flags = fast[k].query & 1;
-            const int lq = seqs[q].first_end;
-            const int strips = (lq + strip - 1) / strip;
-            for (size_t s = k; s < e; s += per_item) {
-                const int32_t cnt = (int32_t)std::min<size_t>(per_item, e - s);
+            const int64_t lq = seqs[q].first_end;
+            const int64_t rows = ((lq + 127) / 128) * 128;
+            for (size_t s = k; s < e; s += kItemTargets) {
+                const int32_t cnt = (int32_t)std::min<size_t>(kItemTargets, e - s);
                 int64_t cost = 0;
-                for (size_t u = s; u < s + cnt; ++u) cost += (int64_t)strips * strip * (seqs[fast[u].target].first_end + lanes - 1);
-                items.push_back({make_int4(q, (int)s, cnt, flags), cost, strips});
+                // A round of four targets costs as much as its longest.
+                for (size_t u = s; u < s + cnt; u += 4)
+                    cost += rows * (((seqs[fast[u].target].first_end + 30) / 16) * 16);
+                items.push_back({make_int4(q, (int)s, cnt, flags), cost});
             }
             k = e;
         }
-        // Bins by strip count: 1, 2, 3-4, 5-8, ... so that the LDS size of
-        // each launch fits its queries and short queries keep occupancy.
-        std::vector<std::vector<Item>> binned;
-        std::vector<int> caps;
-        for (const auto& it : items) {
-            int cap = 1, b = 0;
-            while (cap < it.strips) { cap *= 2; ++b; }
-            if ((int)binned.size() <= b) { binned.resize(b + 1); caps.resize(b + 1, 0); }
-            binned[b].push_back(it);
-            caps[b] = std::max(caps[b], it.strips);
-        }
-        int2* d_targets = nullptr;
-        MSA_TRY(plan.upload(d_targets, targets.data(), targets.size(), stream));
-        for (size_t b = 0; b < binned.size(); ++b) {
-            if (binned[b].empty()) continue;
-            auto& list = binned[b];
-            std::stable_sort(list.begin(), list.end(), [](const Item& x, const Item& y) { return x.cost > y.cost; });
-            Bin bin;
-            bin.lanes = lanes;
-            bin.rows = rows;
-            bin.strips_cap = caps[b];
-            bin.lds = batched_lds_bytes(lanes, rows, bin.strips_cap);
-            bin.nitems = (int32_t)list.size();
-            int per_cu = 0;
-            MSA_TRY(batched_occupancy(lanes, rows, bin.lds, &per_cu));
-            if (per_cu < 1) return fail(MSA_ERR_TOO_LARGE, "batched bin with %d strips does not fit a CU", bin.strips_cap);
-            bin.grid = (int)std::min<int64_t>((int64_t)per_cu * dev.compute_units, bin.nitems);
-            int32_t max_t = 1;
-            std::vector<int4> hitems(list.size());
-            for (size_t u = 0; u < list.size(); ++u) {
-                hitems[u] = list[u].v;
-                for (int32_t v = 0; v < list[u].v.z; ++v)
-                    max_t = std::max(max_t, seqs[targets[list[u].v.y + v].x].first_end);
-            }
-            bin.line_stride = bin.strips_cap > 1 ? ((max_t + 63) & ~63) : 1;
-            MSA_TRY(plan.upload(bin.d_items, hitems.data(), hitems.size(), stream));
-            bin.d_targets = d_targets;
-            MSA_TRY(plan.alloc(bin.d_lines, (size_t)bin.grid * batched_groups_per_block(lanes) * bin.line_stride));
-            MSA_TRY(plan.alloc(bin.d_queue, 1));
-            plan.bins.push_back(bin);
-        }
-        plan.info.bins = (int32_t)plan.bins.size();
+        std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.cost > y.cost; });
+        std::vector<int4> hitems(items.size());
+        for (size_t u = 0; u < items.size(); ++u) hitems[u] = items[u].v;
+        Batch& b = plan.batch;
+        b.nitems = (int32_t)items.size();
+        int per_cu = 0;
+        MSA_TRY(batched_occupancy(&per_cu));
+        if (per_cu < 1) return fail(MSA_ERR_RUNTIME, "batched kernel does not fit a compute unit");
+        b.grid = (int)std::min<int64_t>((int64_t)per_cu * dev.compute_units, (b.nitems + 3) / 4);
+        b.line_stride = (max_t + 63) & ~63;
+        (void)strip_rows;
+        MSA_TRY(plan.upload(b.d_items, hitems.data(), hitems.size(), stream));
+        MSA_TRY(plan.upload(b.d_targets, targets.data(), targets.size(), stream));
+        MSA_TRY(plan.alloc(b.d_lines, (size_t)b.grid * batched_groups_per_block() * b.line_stride));
+        MSA_TRY(plan.alloc(b.d_queue, 1));
+        plan.info.bins = 1;
     }
-    plan.info.launches = (int32_t)(2 * plan.bins.size() + (plan.n_plain ? 1 : 0));
+    plan.info.launches = (int32_t)((plan.batch.nitems ? 2 : 0) + (plan.n_plain ? 1 : 0));
     plan.use_graph = opt.use_graph != 0;
     MSA_HIP(hipStreamSynchronize(stream));
     return MSA_OK;

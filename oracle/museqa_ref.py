@@ -207,3 +207,46 @@ class CpuOracle:
         if rc != 0:
             raise RuntimeError("oracle failed")
         return out
+
+
+class ReferenceBuild:
+    """ctypes wrapper of oracle/_ref/libref_sequential.so: the reference's own
+    sequential.cpp + encoder.cpp compiled from its sources (oracle/Makefile,
+    target `ref`).  Present only where it was built."""
+
+    PATH = os.path.join(HERE, "_ref", "libref_sequential.so")
+
+    @classmethod
+    def available(cls):
+        return os.path.exists(cls.PATH)
+
+    def __init__(self):
+        self.lib = ctypes.CDLL(self.PATH)
+        self.lib.ref_align_list.argtypes = [
+            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+            ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p, ctypes.c_int]
+        self.lib.ref_align_list.restype = ctypes.c_int
+        self.lib.ref_align_pair.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
+                                            ctypes.c_void_p, ctypes.c_float]
+        self.lib.ref_align_pair.restype = ctypes.c_float
+        self.lib.ref_encode.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t]
+        self.lib.ref_encode.restype = ctypes.c_size_t
+
+    def encode(self, text):
+        raw = text.encode("latin-1")
+        out = np.zeros(len(raw) // 3 + 2, dtype=np.uint16)
+        n = self.lib.ref_encode(raw, len(raw), out.ctypes.data, len(out))
+        return [int(v) for v in out[:n]]
+
+    def align_list(self, blocks, offsets, nblocks, pairs, table625, penalty, workers=1):
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint16)
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        nblocks = np.ascontiguousarray(nblocks, dtype=np.int64)
+        pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        table = np.ascontiguousarray(np.asarray(table625, dtype=np.float32).reshape(625))
+        out = np.zeros(len(pairs), dtype=np.float32)
+        if len(pairs):
+            self.lib.ref_align_list(blocks.ctypes.data if len(blocks) else None, offsets.ctypes.data,
+                                    nblocks.ctypes.data, pairs.ctypes.data, len(pairs), table.ctypes.data,
+                                    float(penalty), out.ctypes.data, int(workers))
+        return out

@@ -46,3 +46,46 @@ size_t ref_encode(const char* text, size_t size, uint16_t* out, size_t capacity)
     for (size_t i = 0; i < encoded.size() && i < capacity; ++i) out[i] = encoded[i];
     return encoded.size();
 }
+
+#include <thread>
+#include <vector>
+
+/*
+ * Scores count pairs of an explicit list with the reference's align_pair,
+ * using `workers` threads on contiguous shares (utils::partition, src/utils.hpp:
+ * 176-183), the way each MPI rank takes its range in the reference.  The
+ * orientation is that of align() (sequential.cpp:87-91).  out[c] = score of
+ * pairs[2c], pairs[2c+1].
+ */
+extern "C" __attribute__((visibility("default")))
+int ref_align_list(const uint16_t* blocks, const int64_t* offset, const int64_t* nblocks,
+                   const int32_t* pairs, int64_t count, const float* table625, float penalty,
+                   float* out, int workers)
+{
+    if (workers < 1) workers = 1;
+    std::vector<museqa::sequence> seqs;
+    int64_t nseq = 0;
+    for (int64_t c = 0; c < 2 * count; ++c) nseq = std::max<int64_t>(nseq, pairs[c] + 1);
+    seqs.reserve(nseq);
+    for (int64_t s = 0; s < nseq; ++s) seqs.push_back(make_sequence(blocks + offset[s], (size_t)nblocks[s]));
+    auto work = [&](int64_t first, int64_t total) {
+        using namespace museqa::pairwise;
+        scoring_table::raw_type raw;
+        std::memcpy(&raw, table625, sizeof(raw));
+        const scoring_table table {museqa::pointer<scoring_table::raw_type>::weak(&raw), penalty};
+        for (int64_t c = first; c < first + total; ++c) {
+            const museqa::sequence& one = seqs[pairs[2 * c]];
+            const museqa::sequence& two = seqs[pairs[2 * c + 1]];
+            out[c] = one.size() > two.size() ? align_pair(one, two, table) : align_pair(two, one, table);
+        }
+    };
+    std::vector<std::thread> threads;
+    const int64_t quo = count / workers, rem = count % workers;
+    for (int r = 0; r < workers; ++r) {
+        const int64_t first = quo * r + std::min<int64_t>(rem, r);
+        const int64_t total = quo + (rem > r);
+        threads.emplace_back(work, first, total);
+    }
+    for (auto& t : threads) t.join();
+    return 0;
+}

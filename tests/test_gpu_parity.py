@@ -1,0 +1,240 @@
+"""GPU parity: every kernel, through the C ABI, against the oracle.
+
+Small cases are compared score by score with the CPU oracle (the C
+restatement of sequential.cpp:29-95, itself pinned in test_oracle.py); the
+full-size workloads are checked through properties that do not depend on a
+CPU pass over all pairs: batched == plain on every pair, a random oracle
+sample, and symmetry.  The bar is bit-exact int32 == the reference's float.
+"""
+import zlib
+
+import numpy as np
+import pytest
+
+from msa_amd import _native as N
+from msa_amd import workloads as W
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+KERNELS = {"auto": N.KERNEL_AUTO, "plain": N.KERNEL_PLAIN, "batched": N.KERNEL_BATCHED}
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    if N.device_count() < 1:
+        pytest.skip("no GPU")
+    N.check(N.lib().msa_device_select(0))
+
+
+def gpu_scores(db, T, p, pairs=None, kernel="auto", **kw):
+    return N.align(*db, T, p, pairs=pairs, opts=N.options(KERNELS[kernel], **kw))
+
+
+def assert_oracle(db, T, p, pairs=None, kernel="auto", **kw):
+    got = gpu_scores(db, T, p, pairs=pairs, kernel=kernel, **kw)
+    exp = H.oracle_scores(db, T, p, pairs=pairs)
+    bad = np.nonzero(got.astype(np.float32) != exp)[0]
+    assert len(bad) == 0, "%d/%d differ, first %s: gpu %s oracle %s" % (
+        len(bad), len(exp), bad[:5], got[bad[:5]], exp[bad[:5]])
+    return got
+
+
+@pytest.mark.parametrize("kernel", list(KERNELS))
+@pytest.mark.parametrize("key", sorted(H.golden_scores()))
+def test_reference_fixtures_golden(key, kernel):
+    """test/pairwise.py:13-58: mock1..3 x {default, blosum62, pam250}."""
+    fixture, table = key.split("/")
+    _, db = H.fixture_db(fixture)
+    T, p = H.tables()[table]
+    got = gpu_scores(db, T, p, kernel=kernel)
+    assert [int(v) for v in got] == H.golden_scores()[key]["scores"]
+
+
+@pytest.mark.parametrize("kernel", list(KERNELS))
+@pytest.mark.parametrize("table", ["default", "blosum62", "blosum45", "blosum50", "blosum80", "blosum90", "pam250"])
+def test_random_clean(table, kernel):
+    rng = np.random.default_rng(zlib.crc32((table + kernel).encode()))
+    T, p = H.tables()[table]
+    assert_oracle(H.random_db(rng, 14, 0, 90), T, p, kernel=kernel)
+
+
+@pytest.mark.parametrize("kernel", list(KERNELS))
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_random_padding_codes(seed, kernel):
+    """Internal padding codes: rows stop at the first, columns copy left."""
+    rng = np.random.default_rng(seed)
+    T, p = H.tables()["blosum62"]
+    assert_oracle(H.random_db(rng, 16, 0, 70, dirty=True), T, p, kernel=kernel)
+
+
+EDGE_STRINGS = [
+    "", "A", "AC", "ACG", "ACGT", "*", "**", "***", "*A", "A*", "AC*GT", "ACGT*", "*ACGT", "A*C*G*T",
+    "NNNNXXBJZOU", "acgtMKV", "W" * 65, "GATTACA" * 10, "X\rY", "1234", "MK*" * 30,
+]
+
+
+@pytest.mark.parametrize("kernel", list(KERNELS))
+@pytest.mark.parametrize("table", ["blosum62", "pam250", "default"])
+def test_edge_sequences(table, kernel):
+    T, p = H.tables()[table]
+    assert_oracle(H.strings_db(EDGE_STRINGS), T, p, kernel=kernel)
+
+
+def test_asymmetric_table_keeps_reference_orientation():
+    """Equal block counts put pair.second on the rows (sequential.cpp:87-91);
+    with a non-symmetric table the transposed profile must be used."""
+    rng = np.random.default_rng(9)
+    T, p = H.tables()["blosum62"]
+    Ta = T.copy()
+    Ta[0, 1], Ta[1, 0] = 5, -3
+    Ta[4, 2], Ta[17, 18] = -2, 6
+    db = H.random_db(rng, 24, 1, 120)
+    for kernel in KERNELS:
+        assert_oracle(db, Ta, p, kernel=kernel)
+    db = H.strings_db(["ACGTAC", "CATGCA", "ACG", "CAT"])
+    for kernel in KERNELS:
+        assert_oracle(db, Ta, p, kernel=kernel)
+
+
+@pytest.mark.parametrize("lo,hi", [(1, 130), (120, 400), (380, 700), (500, 1100), (900, 2100)])
+def test_batched_strip_shapes(lo, hi):
+    """Queries of every first-strip height (8..32 rows per lane) and of one to
+    five strips, which exercise the strip boundary line in global memory."""
+    rng = np.random.default_rng(lo + hi)
+    T, p = H.tables()["blosum62"]
+    db = H.random_db(rng, 10, lo, hi)
+    assert_oracle(db, T, p, kernel="batched")
+
+
+def test_plain_multi_strip_with_padding():
+    rng = np.random.default_rng(4)
+    T, p = H.tables()["pam250"]
+    db = H.random_db(rng, 8, 500, 2100, dirty=True)
+    assert_oracle(db, T, p, kernel="plain")
+
+
+def test_explicit_pair_list():
+    rng = np.random.default_rng(5)
+    T, p = H.tables()["blosum62"]
+    db = H.random_db(rng, 30, 0, 200, dirty=True)
+    pairs = rng.integers(0, 30, (300, 2)).astype(np.int32)
+    for kernel in KERNELS:
+        assert_oracle(db, T, p, pairs=pairs, kernel=kernel)
+
+
+def test_uniform_small_config_matches_oracle():
+    """16 protein sequences, L ~ U[100,300], blosum62: all 120 pairs."""
+    w = W.uniform_small(H.tables())
+    for kernel in KERNELS:
+        assert_oracle((w.blocks, w.offsets, w.sizes), w.table, w.penalty, kernel=kernel)
+
+
+def test_graph_replays_match_eager():
+    rng = np.random.default_rng(6)
+    T, p = H.tables()["blosum62"]
+    db = H.random_db(rng, 40, 10, 900, dirty=True)
+    n = 40 * 39 // 2
+    eager = N.Plan(*db, T, p, opts=N.options(N.KERNEL_AUTO, use_graph=False))
+    graph = N.Plan(*db, T, p, opts=N.options(N.KERNEL_AUTO, use_graph=True))
+    out_e = N.DeviceBuffer(4 * n)
+    out_g = N.DeviceBuffer(4 * n)
+    eager.execute(out_e.ptr)
+    N.check(N.lib().msa_device_sync())
+    ref = out_e.download(np.int32, n)
+    assert np.array_equal(ref.astype(np.float32), H.oracle_scores(db, T, p))
+    for _ in range(4):
+        N.check(N.lib().msa_memset(out_g.ptr, 0xAB, 4 * n))
+        graph.execute(out_g.ptr)
+        N.check(N.lib().msa_device_sync())
+        assert np.array_equal(out_g.download(np.int32, n), ref)
+        eager.execute(out_e.ptr)
+        N.check(N.lib().msa_device_sync())
+
+
+def test_plain_entry_on_device_memory():
+    """msa_nw_plain: caller-owned device buffers, all-pairs and explicit list."""
+    lib = N.lib()
+    rng = np.random.default_rng(8)
+    T, p = H.tables()["blosum62"]
+    blocks, offsets, sizes = H.random_db(rng, 20, 0, 700, dirty=True)
+    n = 20
+    d_blocks = N.DeviceBuffer(blocks.nbytes)
+    d_blocks.upload(blocks)
+    d_off = N.DeviceBuffer(offsets.nbytes)
+    d_off.upload(offsets)
+    d_size = N.DeviceBuffer(sizes.nbytes)
+    d_size.upload(sizes)
+    npairs = n * (n - 1) // 2
+    d_out = N.DeviceBuffer(4 * npairs)
+    table = np.ascontiguousarray(T, dtype=np.int32)
+    N.check(lib.msa_nw_plain(d_blocks.ptr, d_off.ptr, d_size.ptr, n, int(sizes.max()), None, npairs,
+                             table.ctypes.data, p, d_out.ptr, None))
+    N.check(lib.msa_device_sync())
+    exp = H.oracle_scores((blocks, offsets, sizes), T, p)
+    assert np.array_equal(d_out.download(np.int32, npairs).astype(np.float32), exp)
+    pairs = rng.integers(0, n, (77, 2)).astype(np.int32)
+    d_pairs = N.DeviceBuffer(pairs.nbytes)
+    d_pairs.upload(pairs)
+    N.check(lib.msa_nw_plain(d_blocks.ptr, d_off.ptr, d_size.ptr, n, int(sizes.max()), d_pairs.ptr, len(pairs),
+                             table.ctypes.data, p, d_out.ptr, None))
+    N.check(lib.msa_device_sync())
+    exp = H.oracle_scores((blocks, offsets, sizes), T, p, pairs=pairs)
+    assert np.array_equal(d_out.download(np.int32, len(pairs)).astype(np.float32), exp)
+
+
+def _full_size_check(w, sample=200, seed=0):
+    db = (w.blocks, w.offsets, w.sizes)
+    batched = gpu_scores(db, w.table, w.penalty, kernel="auto")
+    plain = gpu_scores(db, w.table, w.penalty, kernel="plain")
+    assert np.array_equal(batched, plain)
+    rng = np.random.default_rng(seed)
+    idx = rng.choice(w.npairs, min(sample, w.npairs), replace=False)
+    pairs = np.array([divmod_pair(int(c)) for c in idx], dtype=np.int32)
+    exp = H.oracle_scores(db, w.table, w.penalty, pairs=pairs, workers=16)
+    assert np.array_equal(batched[idx].astype(np.float32), exp)
+    return batched
+
+
+def divmod_pair(c):
+    i = int((1 + np.sqrt(1 + 8 * c)) // 2)
+    while i * (i - 1) // 2 > c:
+        i -= 1
+    while (i + 1) * i // 2 <= c:
+        i += 1
+    return i, c - i * (i - 1) // 2
+
+
+def test_full_uniform_batch_512():
+    """512 sequences, L ~ U[800,1200]: all 130,816 pairs, batched == plain."""
+    _full_size_check(W.uniform_batch(H.tables()))
+
+
+def test_full_uneven_batch_2000():
+    """2000 sequences, log-normal lengths: all 1,999,000 pairs, batched == plain."""
+    _full_size_check(W.uneven_batch(H.tables()), sample=300)
+
+
+def test_long_dna_sample():
+    """64 DNA sequences of 30,000 (+5/-4, gap 8): a pair subset vs the oracle."""
+    w = W.long_dna()
+    rng = np.random.default_rng(1)
+    pairs = rng.choice(64, (12, 2)).astype(np.int32)
+    assert_oracle((w.blocks, w.offsets, w.sizes), w.table, w.penalty, pairs=pairs, kernel="auto")
+
+
+def test_gap_code_rejected_on_device_path_too():
+    T, p = H.tables()["blosum62"]
+    with pytest.raises(N.MsaError) as e:
+        gpu_scores(H.strings_db(["AC-GT", "ACGT"]), T, p)
+    assert e.value.status == N.MSA_ERR_GAP_CODE
+
+
+def test_symmetry_of_scores():
+    """Symmetric tables: swapping the pair members never changes a score."""
+    rng = np.random.default_rng(10)
+    T, p = H.tables()["blosum62"]
+    db = H.random_db(rng, 25, 0, 300)
+    pairs = np.array([(i, j) for i in range(25) for j in range(25)], dtype=np.int32)
+    got = gpu_scores(db, T, p, pairs=pairs).reshape(25, 25)
+    assert (got == got.T).all()
