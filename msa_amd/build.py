@@ -56,8 +56,28 @@ def build_hip(jobs=4):
     return out
 
 
+def build_host():
+    """libmuseqa.so (the C++ host layer) and the _museqa Python module."""
+    host = os.path.join(CSRC, "host")
+    common = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-I", INCLUDE, "-I", host]
+    rpath = "-Wl,-rpath,$ORIGIN"
+    lib = os.path.join(PKG, "libmuseqa.so")
+    _run(["g++"] + common + ["-shared", "-o", lib + ".tmp", os.path.join(host, "museqa.cpp"),
+                              "-L", PKG, "-lmsa_hip", rpath])
+    os.replace(lib + ".tmp", lib)
+    import pybind11
+    suffix = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
+    mod = os.path.join(PKG, "_museqa" + suffix)
+    _run(["g++"] + common + ["-shared", "-fvisibility=hidden", "-I", pybind11.get_include(),
+                              "-I", sysconfig.get_paths()["include"], "-o", mod + ".tmp",
+                              os.path.join(host, "pymuseqa.cpp"), "-L", PKG, "-lmuseqa", "-lmsa_hip", rpath])
+    os.replace(mod + ".tmp", mod)
+    return mod
+
+
 def build_all():
     build_hip()
+    build_host()
 
 
 if __name__ == "__main__":

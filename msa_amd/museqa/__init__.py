@@ -1,0 +1,4 @@
+"""Python mirror of the reference's museqa.sequence / museqa.database /
+museqa.pairwise modules (src/python/*.pyx), over the C++ host layer
+(msa_amd/csrc/host) and the HIP kernels behind libmsa_hip.so."""
+from . import database, pairwise, sequence  # noqa: F401

@@ -1,0 +1,223 @@
+"""The museqa surface (C++ host layer + Python binding), mirroring the
+reference's own tests: test/sequence.py, test/database.py, test/loader.py
+and test/pairwise.py.  The pairwise cases need a GPU and compare every
+registered algorithm with the oracle, as test/pairwise.py:46-66 compares
+'sequential'/'hybrid' with the Python needleman."""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from msa_amd.museqa import database as mdb
+from msa_amd.museqa import pairwise as mpw
+from msa_amd.museqa import sequence as mseq
+from oracle import museqa_ref as M
+from tests import helpers as H
+from tests.test_oracle import SEQUENCE_FIXTURES
+
+Database, Sequence, encode = mdb.Database, mseq.Sequence, mseq.encode
+
+
+# ---- test/sequence.py -------------------------------------------------------
+
+@pytest.fixture(params=SEQUENCE_FIXTURES)
+def sequence(request):
+    return Sequence(request.param), encode(request.param)
+
+
+def testCanInstantiateSequence(sequence):
+    target, expected = sequence
+    assert expected == str(target)
+
+
+def testCanAccessSequenceElement(sequence):
+    target, expected = sequence
+    for i in range(len(expected)):
+        assert expected[i] == target[i]
+
+
+def testIfSequenceHasCorrectLength(sequence):
+    target, expected = sequence
+    assert len(expected) == target.length
+
+
+def testIfRaisesOnIndexError(sequence):
+    target, _ = sequence
+    with pytest.raises(IndexError):
+        target[target.length]
+
+
+def test_encode_matches_oracle_encoder():
+    for text in SEQUENCE_FIXTURES + ["acgt*-x1", "", "NNX\r"]:
+        assert encode(text) == M.decode(M.encode(text))
+        assert str(Sequence(text)) == M.decode(M.encode(text))
+
+
+# ---- test/database.py -------------------------------------------------------
+
+DB_FIXTURES = [
+    {"SQ01": "MNNQRKKTGRPSFNMLKRARNRVSTGSQLAKRFSKGLLSGQGPMKLVMAFIAFLRFLAIPPTAGILA**",
+     "SQ02": "TEVKGYTKGXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXXX**",
+     "SQ03": "MKNPKKKSGGFRIVNMLKRGVARVSPFGGLKRLPAGLLLGHGPIRMVLAILAFLRFTAIKPSLGLIN**",
+     "SQ04": "AAAAATCGATCAAGCAATCTCTCAGTCTCTCAGTTTACAGCAATCTTTGCAAACACACGAACCACAA**",
+     "SQ05": "LLNRFTMAHMKPTYERDVDLGAGTRHVAVEPEVANLDIIGQRIENIKNEHKSTWHYDEDNPYKTWAY**"},
+    {"SQ06": "GATAAAAGAACCTATAATCCCTTCGCACACCGCGTCACACCGCGCTATATGCTGCTCATTAGGAAT",
+     "SQ07": "GAGCAATCGTTACATAACATGCATTCCAAATAAGGGAATGCGGAAATCCATACTTGGGACTTACAT",
+     "SQ08": "GAGCAATCGTTACATAACATGCATTCTAACTAAGGCAATGCGGGATTCCATACTTGGGACTTACAT",
+     "SQ09": "GCTCCTTTTTTGTGGATACAATCTCTTGTATACGATATACTTATTGTTAATTTCATTGACCTTTAC",
+     "SQ10": "TGCATTCCATATAAGGAATACGGATATGCCCATGTTTGTATCCAAACAGGCGGTCTCCCAGACTCC"},
+]
+
+
+@pytest.fixture(params=range(len(DB_FIXTURES)))
+def database(request):
+    return DB_FIXTURES[request.param]
+
+
+@pytest.fixture(params=[2, 3, 4])
+def subsets(request, database):
+    items = [*database.items()]
+    return items[:request.param], items[request.param:]
+
+
+def testIfCanAddSequences(database):
+    db = Database()
+    db.add(database)
+    for name, seq in database.items():
+        assert name == db[name].description
+        assert seq == str(db[name].contents)
+    assert len(database) == db.count
+
+
+def testIfCanMergeDatabases(subsets):
+    dbs = []
+    for i, subset in enumerate(subsets):
+        dbs.append(Database())
+        dbs[i].add(subset)
+        assert len(subset) == dbs[i].count
+    for i in range(1, len(dbs)):
+        dbs[0].merge(dbs[i])
+    for subset in subsets:
+        for name, seq in subset:
+            assert name == dbs[0][name].description
+            assert seq == str(dbs[0][name].contents)
+    assert sum(len(s) for s in subsets) == dbs[0].count
+
+
+def test_only_and_anonymous_entries():
+    db = Database()
+    db.add(["ACGT", "GGCC"])
+    db.add(("named", "TTAA"))
+    assert db.count == 3 and db[0].description.startswith("anonymous#")
+    sub = db.only([2, 0])
+    assert sub.count == 2 and sub[0].description == db[0].description and sub[1].description == "named"
+    with pytest.raises(TypeError):
+        db.only([0, "named"])
+    with pytest.raises(RuntimeError):
+        db[7]
+    with pytest.raises(RuntimeError):
+        db["missing"]
+
+
+# ---- test/loader.py ---------------------------------------------------------
+
+def testLoadFromFastaFile():
+    with open(os.path.join(H.GOLDEN, "loader_md5.json")) as f:
+        digests = json.load(f)["digests"]
+    for fixture, expected in digests.items():
+        db = Database.load(os.path.join(H.GOLDEN, fixture))
+        for key, digest in expected.items():
+            assert key == db[key].description
+            assert digest == hashlib.md5(str(db[key].contents).encode()).hexdigest()
+
+
+def testIfRaisesOnUnknownFile():
+    with pytest.raises(RuntimeError):
+        Database.load("unknown/file/path")
+    with pytest.raises(RuntimeError):
+        Database.load("missing-file.fasta")
+
+
+# ---- pairwise surface (no kernel launch) -------------------------------------
+
+def test_scoring_table_surface():
+    assert mpw.ScoringTable.list() == ["default", "blosum62", "blosum45", "blosum50", "blosum80", "blosum90", "pam250"]
+    t = mpw.ScoringTable("pam250")
+    assert t.penalty == 8 and t["W", "W"] == 17 and t[3, 3] == 4
+    with pytest.raises(RuntimeError, match="unknown pairwise scoring table"):
+        mpw.ScoringTable("nope")
+    with pytest.raises(RuntimeError):
+        t[25, 0]
+
+
+def test_algorithm_registry():
+    names = mpw.list()
+    for n in ["default", "needleman", "hybrid", "needleman-hybrid", "sequential", "distributed",
+              "needleman-sequential", "needleman-distributed", "hip", "hip-plain", "hip-batched", "hip-long"]:
+        assert n in names
+    with pytest.raises(RuntimeError, match="unknown pairwise algorithm 'nope'"):
+        mpw.run(Database(), algorithm="nope")
+
+
+def test_distance_matrix_layout():
+    """[x, y] reads buf[nchoose(max) + min]; the diagonal reads 0 (pairwise.cuh:99-107)."""
+    scores = [float(v) for v in range(10)]
+    d = mpw.DistanceMatrix(scores, 5)
+    assert d.count == 5
+    c = 0
+    for i in range(1, 5):
+        for j in range(i):
+            assert d[i, j] == d[j, i] == scores[c]
+            c += 1
+    assert all(d[i, i] == 0 for i in range(5))
+
+
+def test_python_callable_algorithm_is_accepted():
+    """pairwise.pyx:97-98: a Python function may stand for the algorithm."""
+    db = Database.load(os.path.join(H.GOLDEN, "mock2.fasta"))
+
+    def fake(db, table):
+        return mpw.DistanceMatrix([1.0] * 3, db.count)
+
+    assert mpw.run(db, algorithm=fake)[1, 0] == 1.0
+
+
+# ---- test/pairwise.py (GPU) ---------------------------------------------------
+
+def python_needleman(db, table):
+    """The reference's algorithm.needleman restated (oracle), as a callable."""
+    seqs = [db[i].contents.native.blocks() for i in range(db.count)]
+    T = [[table[x, y] for y in range(25)] for x in range(25)]
+    return mpw.DistanceMatrix(M.needleman(seqs, T, table.penalty), db.count)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fixture", ["mock1.fasta", "mock2.fasta", "mock3.fasta"])
+@pytest.mark.parametrize("table", ["default", "blosum62", "pam250"])
+@pytest.mark.parametrize("algorithm", ["default", "hybrid", "sequential", "hip-batched", "hip-plain"])
+def testNeedlemanAlgorithms(fixture, table, algorithm):
+    db = Database.load(os.path.join(H.GOLDEN, fixture))
+    expected = mpw.run(db, algorithm=python_needleman, table=table)
+    produced = mpw.run(db, algorithm=algorithm, table=table)
+    assert expected.count == produced.count
+    for i in range(expected.count):
+        for j in range(produced.count):
+            assert expected[i, j] == produced[i, j]
+    golden = H.golden_scores()["%s/%s" % (fixture, table)]["scores"]
+    assert [int(v) for v in produced.scores()] == golden
+
+
+@pytest.mark.gpu
+def test_module_pairwise_contract():
+    """module::pairwise::check/run (src/pairwise.cpp:27-55)."""
+    from msa_amd.museqa._core import core
+    db = Database.load(os.path.join(H.GOLDEN, "mock3.fasta"))
+    opts = {"pairwise": "needleman", "scoring-table": "blosum62"}
+    assert core.module_check(opts)
+    with pytest.raises(RuntimeError, match="unknown pairwise algorithm chosen"):
+        core.module_check({"pairwise": "zzz"})
+    out_db, dm, count = core.module_run(opts, db.native)
+    assert count == 5 and dm.count == 5
+    assert [int(v) for v in dm.raw()] == H.golden_scores()["mock3.fasta/blosum62"]["scores"]
