@@ -104,8 +104,11 @@ def pair_index(c):
     return i, c - i * (i - 1) // 2
 
 
-def oracle_sample(w, scores, count, seed=11):
+def oracle_sample(w, scores, count, seed=11, max_cells=2e10):
+    """Checks a random sample of pairs (at most `count`, and about `max_cells`
+    cells of CPU work) against the oracle."""
     from oracle import museqa_ref as M
+    count = int(max(8, min(count, max_cells // max(1.0, w.cells() / w.npairs))))
     idx = np.random.default_rng(seed).choice(w.npairs, min(count, w.npairs), replace=False)
     pairs = np.array([pair_index(int(c)) for c in idx], dtype=np.int32)
     exp = M.CpuOracle().align(w.blocks, w.offsets, w.sizes, w.table.astype(np.float32), w.penalty,

@@ -164,6 +164,10 @@ int msa_nw_plan_create(msa_nw_plan* plan,
  * asynchronously on `stream`.  Inputs are already resident. */
 int msa_nw_plan_execute(msa_nw_plan plan, int32_t* d_out, msa_stream stream);
 int msa_nw_plan_info_get(msa_nw_plan plan, msa_nw_plan_info* info);
+/* Waits for `stream` and reports MSA_ERR_RUNTIME if the last execution's long
+ * kernel gave up waiting on a strip (an internal bound, never reached when
+ * the kernel runs normally). */
+int msa_nw_plan_status(msa_nw_plan plan, msa_stream stream);
 int msa_nw_plan_destroy(msa_nw_plan plan);
 
 /*

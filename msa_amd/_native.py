@@ -87,6 +87,7 @@ SIGNATURES = {
     "msa_nw_plan_execute": (ctypes.c_int, [_vp, _vp, _vp]),
     "msa_nw_plan_info_get": (ctypes.c_int, [_vp, ctypes.POINTER(PlanInfo)]),
     "msa_nw_plan_destroy": (ctypes.c_int, [_vp]),
+    "msa_nw_plan_status": (ctypes.c_int, [_vp, _vp]),
     "msa_nw_align": (ctypes.c_int, [_vp, _vp, _vp, _i32, _vp, _i64, _vp, _i32, ctypes.POINTER(NwOptions), _vp]),
     "msa_nw_plain": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _vp, _i64, _vp, _i32, _vp, _vp]),
 }
@@ -217,6 +218,9 @@ class Plan:
 
     def execute(self, d_out, stream=None):
         check(lib().msa_nw_plan_execute(self.handle, d_out, stream))
+
+    def status(self, stream=None):
+        check(lib().msa_nw_plan_status(self.handle, stream))
 
     def close(self):
         if getattr(self, "handle", None):

@@ -40,6 +40,14 @@ int batched_occupancy(int* blocks_per_cu);
 int batched_launch(int grid, const uint16_t* d_blocks, const SeqInfo* d_info, const int4* d_items, int32_t nitems,
                    const int2* d_targets, int32_t* d_queue, int32_t* d_out, int32_t* d_lines, int32_t line_stride,
                    const Sxx& sxx, int32_t penalty, hipStream_t stream);
+int long_rows_for_slots(int slots);
+int long_strip_rows(int rows);
+size_t long_lds_bytes(int rows, int slots, int max_boundaries);
+int long_occupancy(int rows, size_t lds, int* blocks_per_cu);
+int long_launch(int rows, int grid, size_t lds, const uint16_t* d_blocks, const SeqInfo* d_info, const int4* d_pairs,
+                int32_t npairs, int32_t* d_queue, int32_t* d_error, int32_t* d_out, int32_t* d_lines,
+                int32_t line_stride, int32_t max_boundaries, int slots, const uint8_t* remap,
+                const uint8_t* slot_code, const Sxx& sxx, int32_t penalty, hipStream_t stream);
 
 namespace {
 
@@ -110,6 +118,21 @@ int device_state(DeviceState& st)
     return MSA_OK;
 }
 
+struct LongPart {
+    int rows = 32, grid = 0, slots = 0;
+    size_t lds = 0;
+    int32_t npairs = 0;
+    int32_t line_stride = 1, max_boundaries = 1;
+    uint8_t remap[32] = {};
+    uint8_t slot_code[32] = {};
+    int4* d_pairs = nullptr;
+    int32_t* d_lines = nullptr;
+    int32_t* d_queue = nullptr;
+    int32_t* d_error = nullptr;
+};
+
+constexpr int kDefaultLongThreshold = 8192;
+
 struct Batch {
     int grid = 0;
     int32_t nitems = 0;
@@ -139,6 +162,8 @@ struct msa_nw_plan_opaque {
     int plain_grid = 0;
     // batched part
     msa::Batch batch;
+    // long part
+    msa::LongPart lng;
     // graph
     bool use_graph = false;
     hipStream_t capture_stream = nullptr;
@@ -183,6 +208,10 @@ struct msa_nw_plan_opaque {
         if (batch.nitems)
             MSA_TRY(msa::batched_launch(batch.grid, d_blocks, d_info, batch.d_items, batch.nitems, batch.d_targets,
                                         batch.d_queue, d_out, batch.d_lines, batch.line_stride, sxx, penalty, stream));
+        if (lng.npairs)
+            MSA_TRY(msa::long_launch(lng.rows, lng.grid, lng.lds, d_blocks, d_info, lng.d_pairs, lng.npairs,
+                                     lng.d_queue, lng.d_error, d_out, lng.d_lines, lng.line_stride,
+                                     lng.max_boundaries, lng.slots, lng.remap, lng.slot_code, sxx, penalty, stream));
         if (n_plain)
             MSA_TRY(msa::plain_launch(d_blocks, d_info, d_plain_pairs, d_plain_index, n_plain, sxx, penalty, d_out,
                                       d_plain_lines, plain_line_stride, plain_grid, stream));
@@ -223,7 +252,9 @@ int build_plan(msa_nw_plan_opaque& plan, const uint16_t* blocks, const int64_t* 
     // Classify pairs.
     std::vector<int32_t> plain_pairs, plain_index;
     struct Fast { int32_t query, target, index; };
-    std::vector<Fast> fast;
+    std::vector<Fast> fast, longs;
+    const int64_t long_threshold = kernel == MSA_KERNEL_LONG ? 0
+                                 : (opt.long_threshold > 0 ? opt.long_threshold : kDefaultLongThreshold);
     fast.reserve(kernel == MSA_KERNEL_PLAIN ? 0 : (size_t)npairs);
     int64_t cells = 0;
     int64_t c = 0;
@@ -246,7 +277,10 @@ int build_plan(msa_nw_plan_opaque& plan, const uint16_t* blocks, const int64_t* 
         const int32_t q = row_query ? rowref : colref;
         const int32_t t = row_query ? colref : rowref;
         const int32_t transpose = (!symmetric && !row_query) ? 1 : 0;
-        fast.push_back({q * 2 + transpose, t, (int32_t)idx});
+        if (std::min(r.first_end, col.last_real) >= long_threshold)
+            longs.push_back({q * 2 + transpose, t, (int32_t)idx});
+        else
+            fast.push_back({q * 2 + transpose, t, (int32_t)idx});
         return MSA_OK;
     };
     if (pairs) {
@@ -260,6 +294,7 @@ int build_plan(msa_nw_plan_opaque& plan, const uint16_t* blocks, const int64_t* 
     plan.info.cells = cells;
     plan.info.plain_pairs = (int64_t)plain_index.size();
     plan.info.batched_pairs = (int64_t)fast.size();
+    plan.info.long_pairs = (int64_t)longs.size();
 
     // Upload the sequence store; per-sequence facts come from the device scan
     // so that msa_nw_plain and the plan agree on them.
@@ -337,7 +372,54 @@ int build_plan(msa_nw_plan_opaque& plan, const uint16_t* blocks, const int64_t* 
         MSA_TRY(plan.alloc(b.d_queue, 1));
         plan.info.bins = 1;
     }
-    plan.info.launches = (int32_t)((plan.batch.nitems ? 2 : 0) + (plan.n_plain ? 1 : 0));
+    // Long part: compact alphabet of the codes these pairs use, strip height
+    // from the profile size, one pair per workgroup queued longest first.
+    if (!longs.empty()) {
+        LongPart& L = plan.lng;
+        bool present[32] = {};
+        int32_t max_t = 1, max_q = 1;
+        for (const auto& f : longs) {
+            for (const int32_t sidx : {f.query >> 1, f.target}) {
+                const HostSeq& h = seqs[sidx];
+                for (int32_t pos = 0; pos < h.first_end; ++pos)
+                    present[(blocks[h.offset + pos / 3] >> (1 + 5 * (pos % 3))) & 0x1F] = true;
+            }
+            max_t = std::max(max_t, seqs[f.target].first_end);
+            max_q = std::max(max_q, seqs[f.query >> 1].first_end);
+        }
+        int slots = 0;
+        for (int code = 0; code < 32; ++code) L.slot_code[code] = 0xFF;
+        for (int code = 0; code < kEndCode; ++code)
+            if (present[code]) { L.remap[code] = (uint8_t)slots; L.slot_code[slots] = (uint8_t)code; ++slots; }
+        L.slots = slots + 1;                     // the neutral slot is last
+        for (int code = 0; code < 32; ++code)
+            if (code >= kEndCode || !present[code]) L.remap[code] = (uint8_t)slots;
+        L.rows = long_rows_for_slots(L.slots);
+        const int strip = long_strip_rows(L.rows);
+        L.max_boundaries = std::max(1, (max_q + strip - 1) / strip - 1);
+        L.line_stride = (max_t + 63) & ~63;
+        L.lds = long_lds_bytes(L.rows, L.slots, L.max_boundaries);
+        if (L.lds > dev.lds_per_block)
+            return fail(MSA_ERR_TOO_LARGE, "long pairs need %zu bytes of LDS per workgroup", L.lds);
+        int per_cu = 0;
+        MSA_TRY(long_occupancy(L.rows, L.lds, &per_cu));
+        if (per_cu < 1) return fail(MSA_ERR_RUNTIME, "long kernel does not fit a compute unit");
+        std::vector<std::pair<int64_t, int4>> order;
+        for (const auto& f : longs)
+            order.push_back({(int64_t)seqs[f.query >> 1].first_end * seqs[f.target].first_end,
+                             make_int4(f.query >> 1, f.target, f.index, f.query & 1)});
+        std::stable_sort(order.begin(), order.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+        std::vector<int4> hp(order.size());
+        for (size_t u = 0; u < order.size(); ++u) hp[u] = order[u].second;
+        L.npairs = (int32_t)hp.size();
+        L.grid = (int)std::min<int64_t>((int64_t)per_cu * dev.compute_units, L.npairs);
+        MSA_TRY(plan.upload(L.d_pairs, hp.data(), hp.size(), stream));
+        MSA_TRY(plan.alloc(L.d_lines, (size_t)L.grid * L.max_boundaries * L.line_stride));
+        MSA_TRY(plan.alloc(L.d_queue, 1));
+        MSA_TRY(plan.alloc(L.d_error, 1));
+        MSA_HIP(hipMemsetAsync(L.d_error, 0, sizeof(int32_t), stream));
+    }
+    plan.info.launches = (int32_t)((plan.batch.nitems ? 2 : 0) + (plan.lng.npairs ? 2 : 0) + (plan.n_plain ? 1 : 0));
     plan.use_graph = opt.use_graph != 0;
     MSA_HIP(hipStreamSynchronize(stream));
     return MSA_OK;
@@ -406,6 +488,18 @@ int msa_nw_plan_info_get(msa_nw_plan plan, msa_nw_plan_info* info)
     return MSA_OK;
 }
 
+int msa_nw_plan_status(msa_nw_plan plan, msa_stream stream)
+{
+    if (!plan) return fail(MSA_ERR_INVALID_ARGUMENT, "plan is null");
+    MSA_HIP(hipStreamSynchronize((hipStream_t)stream));
+    if (plan->lng.d_error) {
+        int32_t err = 0;
+        MSA_HIP(hipMemcpy(&err, plan->lng.d_error, sizeof(err), hipMemcpyDeviceToHost));
+        if (err) return fail(MSA_ERR_RUNTIME, "long kernel: a strip waited past its limit for the strip above");
+    }
+    return MSA_OK;
+}
+
 int msa_nw_plan_destroy(msa_nw_plan plan)
 {
     delete plan;
@@ -424,6 +518,7 @@ int msa_nw_align(const uint16_t* blocks, const int64_t* offset, const int32_t* s
     if (npairs > 0) {
         rc = msa::hip_status(hipMalloc(&d_out, (size_t)npairs * sizeof(int32_t)), "hipMalloc");
         if (rc == MSA_OK) rc = msa_nw_plan_execute(plan, d_out, nullptr);
+        if (rc == MSA_OK) rc = msa_nw_plan_status(plan, nullptr);
         if (rc == MSA_OK)
             rc = msa::hip_status(hipMemcpy(out, d_out, (size_t)npairs * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy");
         if (d_out) (void)hipFree(d_out);

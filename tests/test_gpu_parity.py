@@ -17,7 +17,7 @@ from tests import helpers as H
 
 pytestmark = pytest.mark.gpu
 
-KERNELS = {"auto": N.KERNEL_AUTO, "plain": N.KERNEL_PLAIN, "batched": N.KERNEL_BATCHED}
+KERNELS = {"auto": N.KERNEL_AUTO, "plain": N.KERNEL_PLAIN, "batched": N.KERNEL_BATCHED, "long": N.KERNEL_LONG}
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -105,6 +105,27 @@ def test_batched_strip_shapes(lo, hi):
     T, p = H.tables()["blosum62"]
     db = H.random_db(rng, 10, lo, hi)
     assert_oracle(db, T, p, kernel="batched")
+
+
+@pytest.mark.parametrize("lo,hi,alphabet", [(1, 300, 24), (400, 3000, 24), (2000, 9000, 4), (100, 5000, 4),
+                                             (5000, 9000, 20)])
+def test_long_kernel_shapes(lo, hi, alphabet):
+    """The multi-wave pipelined kernel: strips of 64R rows (R from the
+    compact alphabet: 32 for DNA, 8 for protein), more strips than waves,
+    boundary rows handed down through global memory."""
+    rng = np.random.default_rng(lo * 7 + hi + alphabet)
+    T, p = H.tables()["blosum62" if alphabet > 4 else "pam250"]
+    db = H.random_db(rng, 7, lo, hi, alphabet=alphabet)
+    assert_oracle(db, T, p, kernel="long")
+
+
+def test_long_kernel_asymmetric_and_padding_routing():
+    rng = np.random.default_rng(77)
+    T, p = H.tables()["blosum62"]
+    Ta = T.copy()
+    Ta[0, 1], Ta[1, 0], Ta[2, 3] = 5, -3, 7
+    db = H.random_db(rng, 8, 600, 2600, dirty=True, alphabet=4)
+    assert_oracle(db, Ta, p, kernel="long")
 
 
 def test_plain_multi_strip_with_padding():
