@@ -166,8 +166,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the pair list would shard by cell "
-              "count across GPUs, see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path here (the pair list would be sharded by "
+              "cell count, one process per GPU); run with --gpus 1", file=sys.stderr)
         return 2
 
     from msa_amd import _native as N
@@ -212,7 +212,8 @@ def main():
             "workload": w.name, "description": w.description, "sequences": w.nseq, "pairs": w.npairs,
             "cells": cells, "input_checksum": w.checksum(), "kernel": "batched persistent (auto routing)",
             "launches_per_step": int(info.launches), "bins": int(info.bins),
-            "batched_pairs": int(info.batched_pairs), "plain_routed_pairs": int(info.plain_pairs),
+            "batched_pairs": int(info.batched_pairs), "long_pairs": int(info.long_pairs),
+            "plain_routed_pairs": int(info.plain_pairs),
             "graph": True, "parallelism": "single GPU",
         },
         "plain_gcups": round(cells / plain["ms"] / 1e6, 2),
