@@ -153,6 +153,68 @@ def cpu_baseline(w, seconds, workers, seed=12):
                       "on contiguous shares as the reference's MPI ranks take them" % (n, cells, dt)}
 
 
+def serving_run(N, tables, rates, seconds, pool_size=48, slots=4, seed=5, log_fn=log):
+    """Poisson job streams through the serving loop.
+
+    Jobs: 8..256 protein sequences (count uniform), lengths uniform in
+    [100, 2000], BLOSUM62 background residues, blosum62 (seed 5); a pool of
+    `pool_size` jobs is cycled.  Arrivals: Poisson at each rate (seed 50).
+    Latency = scores on the host - arrival (submit call).  Every job's scores
+    are compared with the same job scored alone through msa_nw_align.
+    """
+    import threading
+    from msa_amd import workloads as W
+    pool = W.serving_jobs(tables, pool_size, seed=seed)
+    T, p = tables["blosum62"]
+    alone = [N.align(j.blocks, j.offsets, j.sizes, T, p) for j in pool]
+    out = []
+    for rate in rates:
+        count = max(8, int(rate * seconds))
+        arrivals = W.poisson_arrivals(rate, count, seed=50)
+        srv = N.Server(T, p, slots=slots, max_length=2048, device_bytes=4 << 30, host_bytes=512 << 20)
+        ids = [None] * count
+        errors = []
+        t0 = time.time()
+
+        def submitter():
+            try:
+                for k in range(count):
+                    delay = arrivals[k] - (time.time() - t0)
+                    if delay > 0:
+                        time.sleep(delay)
+                    j = pool[k % pool_size]
+                    ids[k] = srv.submit(j.blocks, j.offsets, j.sizes)
+            except Exception as e:  # surfaced below
+                errors.append(e)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        lat, cells, exact, first, last = [], 0, True, None, None
+        for k in range(count):
+            while ids[k] is None and th.is_alive():
+                time.sleep(0.0002)
+            if ids[k] is None:
+                break
+            srv.wait(ids[k], 120000)
+            scores, st = srv.fetch(ids[k])
+            exact = exact and bool(np.array_equal(scores, alone[k % pool_size]))
+            lat.append((st["finished_us"] - st["arrival_us"]) / 1e3)
+            cells += st["cells"]
+            first = st["arrival_us"] if first is None else min(first, st["arrival_us"])
+            last = st["finished_us"] if last is None else max(last, st["finished_us"])
+        th.join()
+        srv.close()
+        if errors:
+            raise errors[0]
+        span_s = max((last - first) / 1e6, 1e-9)
+        rec = {"rate": rate, "jobs": len(lat), "jobs_per_s": round(len(lat) / span_s, 1),
+               "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+               "gcups": round(cells / span_s / 1e9, 1), "bit_exact_vs_alone": exact}
+        log_fn("serving rate %g: %s" % (rate, rec))
+        out.append(rec)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +225,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--oracle-pairs", type=int, default=400)
+    ap.add_argument("--serving", action="store_true", help="only the serving sweep")
+    ap.add_argument("--rates", default="10,20,50,100,200,500")
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-rate", type=float, default=100.0, help="rate of the serving figure in the main line")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -174,6 +240,16 @@ def main():
     if N.device_count() < 1:
         raise SystemExit("no GPU visible")
     tables = {n: N.table(n) for n in N.table_names()}
+    if args.serving:
+        rates = [float(r) for r in args.rates.split(",")]
+        sweep = serving_run(N, tables, rates, args.serve_seconds)
+        print(json.dumps({
+            "metric": METRIC, "value": max(r["jobs_per_s"] for r in sweep), "unit": "jobs/s",
+            "n_gpus": 1, "higher_is_better": True, "data": "synthetic", "dtype": "int32",
+            "config": {"workload": "serving: Poisson jobs of 8-256 protein sequences, L~U[100,2000], blosum62",
+                       "seconds_per_rate": args.serve_seconds, "slots": 4},
+            "serving": sweep}), flush=True)
+        return 0 if all(r["bit_exact_vs_alone"] for r in sweep) else 1
     w = make_workload(args.workload, tables)
     cells = w.cells()
     log("workload:", w.description, "pairs", w.npairs, "cells %.4e" % cells, "checksum", w.checksum())
@@ -226,12 +302,15 @@ def main():
                      "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
                      "ops_per_cell": OPS_PER_CELL},
     }
+    if args.serve_rate > 0:
+        result["serving"] = serving_run(N, tables, [args.serve_rate], args.serve_seconds)[0]
     if not args.no_cpu:
         workers = min(16, os.cpu_count() or 1)
         result["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds, workers)
         result["vs_cpu"] = round(gcups / result["cpu_baseline"]["value"], 1)
     print(json.dumps(result), flush=True)
-    return 0 if (same and oracle_ok) else 1
+    ok = same and oracle_ok and result.get("serving", {}).get("bit_exact_vs_alone", True)
+    return 0 if ok else 1
 
 
 if __name__ == "__main__":

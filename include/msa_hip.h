@@ -195,6 +195,58 @@ int msa_nw_plain(const uint16_t* d_blocks, const int64_t* d_offset, const int32_
                  const int32_t* table625, int32_t penalty,
                  int32_t* d_out, msa_stream stream);
 
+/* ---- serving -----------------------------------------------------------------
+ * No counterpart in the reference (a batch MPI pipeline); its closest relative
+ * is the free-memory admission of hybrid.cu:345-375.  A server keeps a device
+ * arena for admitted jobs' data and scores, a pinned host arena for their
+ * transfers, and `slots` HIP streams, each with its own kernel scratch.  Jobs
+ * are all-pairs alignments of 2..max_sequences sequences of at most
+ * max_length residues, planned on the submitting thread and admitted in
+ * arrival order whenever a slot is free and the job fits the arena; jobs on
+ * different slots run on the device at the same time.  Thread-safe. */
+typedef struct msa_server_opaque* msa_server;
+
+typedef struct {
+    const int32_t* table625;    /* scoring table (copied at open) */
+    int32_t penalty;
+    int32_t slots;              /* jobs on the device at once (0 = 4) */
+    int32_t max_length;         /* longest sequence a job may hold, residues (0 = 8192) */
+    int32_t max_sequences;      /* largest job (0 = 4096) */
+    size_t device_bytes;        /* HBM arena for admitted jobs (0 = 1 GiB) */
+    size_t host_bytes;          /* pinned staging arena (0 = 256 MiB) */
+    msa_nw_options options;     /* routing for every job */
+} msa_srv_config;
+
+typedef struct {
+    double arrival_us;          /* microseconds since msa_srv_open */
+    double admitted_us;
+    double finished_us;         /* scores on the host */
+    int64_t pairs;
+    int64_t cells;
+    int64_t device_bytes;       /* arena bytes the job held */
+    int32_t slot;
+} msa_srv_job_stats;
+
+typedef struct {
+    int64_t submitted, completed, running, pending;
+    int64_t cells;              /* cells of completed jobs */
+    int64_t device_bytes_used, device_bytes_capacity;
+} msa_srv_stats;
+
+int msa_srv_open(msa_server* srv, const msa_srv_config* cfg);
+/* Copies the job's sequences and returns at once; MSA_ERR_TOO_LARGE if the job
+ * can never fit the configuration. */
+int msa_srv_submit(msa_server srv, const uint16_t* blocks, const int64_t* offset, const int32_t* size, int32_t nseq,
+                   uint64_t* job_id);
+int msa_srv_poll(msa_server srv, uint64_t job_id, int* done);
+/* Waits up to timeout_ms (negative: forever); MSA_ERR_NOT_READY on timeout. */
+int msa_srv_wait(msa_server srv, uint64_t job_id, int32_t timeout_ms);
+/* Copies the scores (pair order) of a finished job, its timing, and forgets it. */
+int msa_srv_fetch(msa_server srv, uint64_t job_id, int32_t* out, int64_t capacity, msa_srv_job_stats* stats);
+int msa_srv_stats_get(msa_server srv, msa_srv_stats* stats);
+/* Finishes the running jobs, drops the queued ones, frees everything. */
+int msa_srv_close(msa_server srv);
+
 #ifdef __cplusplus
 }
 #endif

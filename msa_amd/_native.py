@@ -53,6 +53,22 @@ class PlanInfo(ctypes.Structure):
                 ("long_pairs", _i64), ("launches", _i32), ("bins", _i32), ("device_bytes", _i64)]
 
 
+class SrvConfig(ctypes.Structure):
+    _fields_ = [("table625", _vp), ("penalty", _i32), ("slots", _i32), ("max_length", _i32),
+                ("max_sequences", _i32), ("device_bytes", _sz), ("host_bytes", _sz), ("options", NwOptions)]
+
+
+class SrvJobStats(ctypes.Structure):
+    _fields_ = [("arrival_us", ctypes.c_double), ("admitted_us", ctypes.c_double),
+                ("finished_us", ctypes.c_double), ("pairs", _i64), ("cells", _i64),
+                ("device_bytes", _i64), ("slot", _i32)]
+
+
+class SrvStats(ctypes.Structure):
+    _fields_ = [("submitted", _i64), ("completed", _i64), ("running", _i64), ("pending", _i64),
+                ("cells", _i64), ("device_bytes_used", _i64), ("device_bytes_capacity", _i64)]
+
+
 # name -> (restype, argtypes); every symbol include/msa_hip.h declares.
 SIGNATURES = {
     "msa_error_string": (ctypes.c_char_p, [ctypes.c_int]),
@@ -90,6 +106,13 @@ SIGNATURES = {
     "msa_nw_plan_status": (ctypes.c_int, [_vp, _vp]),
     "msa_nw_align": (ctypes.c_int, [_vp, _vp, _vp, _i32, _vp, _i64, _vp, _i32, ctypes.POINTER(NwOptions), _vp]),
     "msa_nw_plain": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _vp, _i64, _vp, _i32, _vp, _vp]),
+    "msa_srv_open": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(SrvConfig)]),
+    "msa_srv_submit": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, ctypes.POINTER(ctypes.c_uint64)]),
+    "msa_srv_poll": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]),
+    "msa_srv_wait": (ctypes.c_int, [_vp, ctypes.c_uint64, _i32]),
+    "msa_srv_fetch": (ctypes.c_int, [_vp, ctypes.c_uint64, _vp, _i64, ctypes.POINTER(SrvJobStats)]),
+    "msa_srv_stats_get": (ctypes.c_int, [_vp, ctypes.POINTER(SrvStats)]),
+    "msa_srv_close": (ctypes.c_int, [_vp]),
 }
 
 
@@ -251,3 +274,60 @@ def align(blocks, offsets, sizes, table625, penalty, pairs=None, opts=None):
     check(lib().msa_nw_align(_ptr(blocks), _ptr(offsets), _ptr(sizes), n, _ptr(pairs), npairs,
                              table625.ctypes.data, int(penalty), ctypes.byref(o), _ptr(out)))
     return out
+
+
+class Server:
+    """msa_srv_*: a serving loop that admits all-pairs jobs as they arrive."""
+
+    def __init__(self, table625, penalty, slots=4, max_length=2048, max_sequences=4096,
+                 device_bytes=1 << 30, host_bytes=256 << 20, opts=None):
+        self._table = np.ascontiguousarray(np.asarray(table625).reshape(625), dtype=np.int32)
+        cfg = SrvConfig(self._table.ctypes.data, int(penalty), int(slots), int(max_length), int(max_sequences),
+                        int(device_bytes), int(host_bytes), opts or options())
+        h = _vp()
+        check(lib().msa_srv_open(ctypes.byref(h), ctypes.byref(cfg)))
+        self.handle = h.value
+        self.npairs = {}
+
+    def submit(self, blocks, offsets, sizes):
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint16)
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+        job = ctypes.c_uint64(0)
+        check(lib().msa_srv_submit(self.handle, _ptr(blocks), _ptr(offsets), _ptr(sizes), len(sizes),
+                                   ctypes.byref(job)))
+        n = len(sizes)
+        self.npairs[job.value] = n * (n - 1) // 2
+        return job.value
+
+    def done(self, job):
+        flag = ctypes.c_int(0)
+        check(lib().msa_srv_poll(self.handle, job, ctypes.byref(flag)))
+        return bool(flag.value)
+
+    def wait(self, job, timeout_ms=-1):
+        check(lib().msa_srv_wait(self.handle, job, int(timeout_ms)))
+
+    def fetch(self, job):
+        """(int32 scores in pair order, stats dict) of a finished job."""
+        n = self.npairs.pop(job)
+        out = np.zeros(n, dtype=np.int32)
+        st = SrvJobStats()
+        check(lib().msa_srv_fetch(self.handle, job, _ptr(out), n, ctypes.byref(st)))
+        return out, {f: getattr(st, f) for f, _ in SrvJobStats._fields_}
+
+    def stats(self):
+        st = SrvStats()
+        check(lib().msa_srv_stats_get(self.handle, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in SrvStats._fields_}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().msa_srv_close(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -3,55 +3,33 @@
 // The planner replaces the reference's host batching (src/pairwise/needleman/
 // impl/hybrid.cu:262-429: required_memory, needed_cache, load_input,
 // make_input, align).  Where the reference cuts the pair list into batches
-// that fit free device memory and launches one thread block per pair, the
-// plan keeps the whole sequence store and every work list resident in HBM,
-// sorts pairs by shape and issues one persistent launch per length bin,
+// that fit free device memory and launches one thread block per pair, a plan
+// keeps the whole sequence store and every work list resident in HBM, orders
+// the work longest first and issues one persistent launch per kernel family,
 // optionally captured in a hipGraph.
 //
-// Routing (every score keeps the reference's sequential semantics):
-//   - batched kernel: pairs whose column sequence (reference orientation,
-//     sequential.cpp:87-91) has no padding code before its last residue and
-//     whose row sequence has at least one row; their score is the global
-//     alignment of the two sequences cut at their first padding code;
-//   - plain kernel: every other pair (internal padding codes, empty sides),
-//     and every pair when MSA_KERNEL_PLAIN is asked for.
-#include "msa_internal.hpp"
+// Routing (every score keeps the reference's sequential semantics,
+// sequential.cpp:29-95):
+//   - plain kernel: pairs whose column sequence (reference orientation,
+//     :87-91) holds a padding code before its last residue, or whose row
+//     sequence runs no row, or whose column sequence has no residue; and every
+//     pair under MSA_KERNEL_PLAIN;
+//   - long kernel: other pairs whose shorter side reaches the long threshold;
+//   - batched kernel: the rest.  For these pairs the reference's rules reduce
+//     to the global alignment of the two sequences cut at their first padding
+//     code; the shorter one is the query.
+#include "nw_planner.hpp"
 
 #include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <new>
-#include <numeric>
-#include <vector>
+#include <utility>
 
 namespace msa {
 
-int plain_launch(const uint16_t* d_blocks, const SeqInfo* d_info, const int32_t* d_pairs,
-                 const int32_t* d_out_index, int64_t npairs, const Sxx& sxx, int32_t penalty,
-                 int32_t* d_out, int32_t* d_lines, int32_t line_stride, int grid, hipStream_t stream);
-int plain_grid(int64_t npairs, int compute_units);
-int seq_info_launch(const uint16_t* d_blocks, const int64_t* d_offset, const int32_t* d_size, int32_t nseq,
-                    SeqInfo* d_info, hipStream_t stream);
-int batched_threads();
-size_t batched_lds_bytes();
-int batched_groups_per_block();
-int batched_strip_rows();
-int batched_occupancy(int* blocks_per_cu);
-int batched_launch(int grid, const uint16_t* d_blocks, const SeqInfo* d_info, const int4* d_items, int32_t nitems,
-                   const int2* d_targets, int32_t* d_queue, int32_t* d_out, int32_t* d_lines, int32_t line_stride,
-                   const Sxx& sxx, int32_t penalty, hipStream_t stream);
-int long_rows_for_slots(int slots);
-int long_strip_rows(int rows);
-size_t long_lds_bytes(int rows, int slots, int max_boundaries);
-int long_occupancy(int rows, size_t lds, int* blocks_per_cu);
-int long_launch(int rows, int grid, size_t lds, const uint16_t* d_blocks, const SeqInfo* d_info, const int4* d_pairs,
-                int32_t npairs, int32_t* d_queue, int32_t* d_error, int32_t* d_out, int32_t* d_lines,
-                int32_t line_stride, int32_t max_boundaries, int slots, const uint8_t* remap,
-                const uint8_t* slot_code, const Sxx& sxx, int32_t penalty, hipStream_t stream);
-
 namespace {
 
-// Host view of one sequence (the same facts as SeqInfo).
 struct HostSeq {
     int64_t offset;
     int32_t blocks;
@@ -86,6 +64,10 @@ int scan_sequences(const uint16_t* blocks, const int64_t* offset, const int32_t*
     return MSA_OK;
 }
 
+size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+}  // namespace
+
 int make_sxx(const int32_t* table625, int32_t penalty, Sxx& sxx, bool& symmetric)
 {
     std::memset(&sxx, 0, sizeof(sxx));
@@ -102,132 +84,30 @@ int make_sxx(const int32_t* table625, int32_t penalty, Sxx& sxx, bool& symmetric
     return MSA_OK;
 }
 
-struct DeviceState {
-    int compute_units = 0;
-    size_t lds_per_block = 0;
-};
-
-int device_state(DeviceState& st)
+int device_shape(DeviceShape& out)
 {
+    static std::mutex lock;
+    static DeviceShape cache[64];
     int dev = 0;
     MSA_HIP(hipGetDevice(&dev));
-    hipDeviceProp_t p;
-    MSA_HIP(hipGetDeviceProperties(&p, dev));
-    st.compute_units = p.multiProcessorCount;
-    st.lds_per_block = p.sharedMemPerBlock;
+    if (dev < 0 || dev >= 64) return fail(MSA_ERR_INVALID_ARGUMENT, "device %d out of range", dev);
+    std::lock_guard<std::mutex> guard(lock);
+    if (!cache[dev].compute_units) {
+        hipDeviceProp_t p;
+        MSA_HIP(hipGetDeviceProperties(&p, dev));
+        cache[dev].compute_units = p.multiProcessorCount;
+        cache[dev].lds_per_block = p.sharedMemPerBlock;
+    }
+    out = cache[dev];
     return MSA_OK;
 }
 
-struct LongPart {
-    int rows = 32, grid = 0, slots = 0;
-    size_t lds = 0;
-    int32_t npairs = 0;
-    int32_t line_stride = 1, max_boundaries = 1;
-    uint8_t remap[32] = {};
-    uint8_t slot_code[32] = {};
-    int4* d_pairs = nullptr;
-    int32_t* d_lines = nullptr;
-    int32_t* d_queue = nullptr;
-    int32_t* d_error = nullptr;
-};
-
-constexpr int kDefaultLongThreshold = 8192;
-
-struct Batch {
-    int grid = 0;
-    int32_t nitems = 0;
-    int32_t line_stride = 1;
-    int4* d_items = nullptr;
-    int2* d_targets = nullptr;
-    int32_t* d_lines = nullptr;
-    int32_t* d_queue = nullptr;
-};
-
-}  // namespace
-}  // namespace msa
-
-struct msa_nw_plan_opaque {
-    int device = 0;
-    int32_t penalty = 0;
-    msa::Sxx sxx;
-    msa_nw_plan_info info{};
-    uint16_t* d_blocks = nullptr;
-    msa::SeqInfo* d_info = nullptr;
-    // plain part
-    int64_t n_plain = 0;
-    int32_t* d_plain_pairs = nullptr;
-    int32_t* d_plain_index = nullptr;
-    int32_t* d_plain_lines = nullptr;
-    int32_t plain_line_stride = 1;
-    int plain_grid = 0;
-    // batched part
-    msa::Batch batch;
-    // long part
-    msa::LongPart lng;
-    // graph
-    bool use_graph = false;
-    hipStream_t capture_stream = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    int32_t* graph_out = nullptr;
-    std::vector<void*> allocations;
-
-    ~msa_nw_plan_opaque()
-    {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        if (capture_stream) (void)hipStreamDestroy(capture_stream);
-        for (void* p : allocations) (void)hipFree(p);
-    }
-
-    template <typename T>
-    int alloc(T*& ptr, size_t count)
-    {
-        void* p = nullptr;
-        MSA_HIP(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
-        allocations.push_back(p);
-        info.device_bytes += (int64_t)(std::max<size_t>(count, 1) * sizeof(T));
-        ptr = static_cast<T*>(p);
-        return MSA_OK;
-    }
-
-    template <typename T>
-    int upload(T*& ptr, const T* host, size_t count, hipStream_t stream)
-    {
-        MSA_TRY(alloc(ptr, count));
-        // The host arrays are temporaries of the planner: copy synchronously.
-        if (count) {
-            MSA_HIP(hipMemcpyAsync(ptr, host, count * sizeof(T), hipMemcpyHostToDevice, stream));
-            MSA_HIP(hipStreamSynchronize(stream));
-        }
-        return MSA_OK;
-    }
-
-    int launch_all(int32_t* d_out, hipStream_t stream)
-    {
-        if (batch.nitems)
-            MSA_TRY(msa::batched_launch(batch.grid, d_blocks, d_info, batch.d_items, batch.nitems, batch.d_targets,
-                                        batch.d_queue, d_out, batch.d_lines, batch.line_stride, sxx, penalty, stream));
-        if (lng.npairs)
-            MSA_TRY(msa::long_launch(lng.rows, lng.grid, lng.lds, d_blocks, d_info, lng.d_pairs, lng.npairs,
-                                     lng.d_queue, lng.d_error, d_out, lng.d_lines, lng.line_stride,
-                                     lng.max_boundaries, lng.slots, lng.remap, lng.slot_code, sxx, penalty, stream));
-        if (n_plain)
-            MSA_TRY(msa::plain_launch(d_blocks, d_info, d_plain_pairs, d_plain_index, n_plain, sxx, penalty, d_out,
-                                      d_plain_lines, plain_line_stride, plain_grid, stream));
-        return MSA_OK;
-    }
-};
-
-namespace msa {
-namespace {
-
-int build_plan(msa_nw_plan_opaque& plan, const uint16_t* blocks, const int64_t* offset, const int32_t* size,
-               int32_t nseq, const int32_t* pairs, int64_t npairs, const int32_t* table625, int32_t penalty,
-               const msa_nw_options& opt, hipStream_t stream)
+int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const int32_t* size, int32_t nseq,
+              const int32_t* pairs, int64_t npairs, const int32_t* table625, int32_t penalty,
+              const msa_nw_options& opt)
 {
     if (nseq < 0 || npairs < 0) return fail(MSA_ERR_INVALID_ARGUMENT, "negative sizes");
-    if (nseq > 0 && (!offset || !size)) return fail(MSA_ERR_INVALID_ARGUMENT, "null sequence views");
+    if (nseq > 0 && (!offset || !size || !blocks)) return fail(MSA_ERR_INVALID_ARGUMENT, "null sequence database");
     if (!table625) return fail(MSA_ERR_INVALID_ARGUMENT, "null table");
     if (!pairs && npairs != (int64_t)nseq * (nseq - 1) / 2)
         return fail(MSA_ERR_INVALID_ARGUMENT, "all-pairs mode needs npairs = nseq*(nseq-1)/2 (%lld), got %lld",
@@ -242,22 +122,16 @@ int build_plan(msa_nw_plan_opaque& plan, const uint16_t* blocks, const int64_t* 
     int64_t total_blocks = 0;
     MSA_TRY(scan_sequences(blocks, offset, size, nseq, seqs, total_blocks));
     bool symmetric = true;
-    MSA_TRY(make_sxx(table625, penalty, plan.sxx, symmetric));
-    plan.penalty = penalty;
+    MSA_TRY(make_sxx(table625, penalty, hp.sxx, symmetric));
+    hp.penalty = penalty;
 
-    DeviceState dev;
-    MSA_TRY(device_state(dev));
-    MSA_HIP(hipGetDevice(&plan.device));
-
-    // Classify pairs.
-    std::vector<int32_t> plain_pairs, plain_index;
+    // Classify pairs (host only).
     struct Fast { int32_t query, target, index; };
     std::vector<Fast> fast, longs;
+    fast.reserve(kernel == MSA_KERNEL_PLAIN ? 0 : (size_t)npairs);
     const int64_t long_threshold = kernel == MSA_KERNEL_LONG ? 0
                                  : (opt.long_threshold > 0 ? opt.long_threshold : kDefaultLongThreshold);
-    fast.reserve(kernel == MSA_KERNEL_PLAIN ? 0 : (size_t)npairs);
     int64_t cells = 0;
-    int64_t c = 0;
     auto visit = [&](int32_t first, int32_t second, int64_t idx) -> int {
         if (first < 0 || first >= nseq || second < 0 || second >= nseq)
             return fail(MSA_ERR_INVALID_ARGUMENT, "pair %lld names sequence outside 0..%d", (long long)idx, nseq - 1);
@@ -268,9 +142,9 @@ int build_plan(msa_nw_plan_opaque& plan, const uint16_t* blocks, const int64_t* 
         cells += (int64_t)r.first_end * col.last_real;
         const bool clean = col.first_end == col.last_real && r.first_end > 0 && col.last_real > 0;
         if (kernel == MSA_KERNEL_PLAIN || !clean) {
-            plain_pairs.push_back(first);
-            plain_pairs.push_back(second);
-            plain_index.push_back((int32_t)idx);
+            hp.plain_pairs.push_back(first);
+            hp.plain_pairs.push_back(second);
+            hp.plain_index.push_back((int32_t)idx);
             return MSA_OK;
         }
         const bool row_query = r.first_end < col.last_real || (r.first_end == col.last_real && rowref < colref);
@@ -284,42 +158,41 @@ int build_plan(msa_nw_plan_opaque& plan, const uint16_t* blocks, const int64_t* 
         return MSA_OK;
     };
     if (pairs) {
-        for (c = 0; c < npairs; ++c) MSA_TRY(visit(pairs[2 * c], pairs[2 * c + 1], c));
+        for (int64_t c = 0; c < npairs; ++c) MSA_TRY(visit(pairs[2 * c], pairs[2 * c + 1], c));
     } else {
+        int64_t c = 0;
         for (int32_t i = 1; i < nseq; ++i)
             for (int32_t j = 0; j < i; ++j, ++c) MSA_TRY(visit(i, j, c));
     }
+    hp.info.pairs = npairs;
+    hp.info.cells = cells;
+    hp.info.plain_pairs = (int64_t)hp.plain_index.size();
+    hp.info.batched_pairs = (int64_t)fast.size();
+    hp.info.long_pairs = (int64_t)longs.size();
 
-    plan.info.pairs = npairs;
-    plan.info.cells = cells;
-    plan.info.plain_pairs = (int64_t)plain_index.size();
-    plan.info.batched_pairs = (int64_t)fast.size();
-    plan.info.long_pairs = (int64_t)longs.size();
+    DeviceShape dev;
+    MSA_TRY(device_shape(dev));
 
-    // Upload the sequence store; per-sequence facts come from the device scan
-    // so that msa_nw_plain and the plan agree on them.
-    std::vector<SeqInfo> hinfo(nseq);
+    // Sequence store, with the views rebased as the reference's to_device does
+    // (src/pairwise/database.cu:56-72).
+    hp.blocks.assign(blocks, blocks + total_blocks);
+    hp.seqinfo.resize(nseq);
     for (int32_t s = 0; s < nseq; ++s)
-        hinfo[s] = make_int4((int)seqs[s].offset, seqs[s].blocks, seqs[s].first_end, seqs[s].last_real);
-    MSA_TRY(plan.upload(plan.d_blocks, blocks, (size_t)total_blocks, stream));
-    MSA_TRY(plan.upload(plan.d_info, hinfo.data(), hinfo.size(), stream));
+        hp.seqinfo[s] = make_int4((int)seqs[s].offset, seqs[s].blocks, seqs[s].first_end, seqs[s].last_real);
 
-    // Plain part.
-    if (!plain_index.empty()) {
-        plan.n_plain = (int64_t)plain_index.size();
-        MSA_TRY(plan.upload(plan.d_plain_pairs, plain_pairs.data(), plain_pairs.size(), stream));
-        MSA_TRY(plan.upload(plan.d_plain_index, plain_index.data(), plain_index.size(), stream));
+    // Plain part: one line per workgroup, as long as the longest column range.
+    if (!hp.plain_index.empty()) {
         int32_t max_cols = 1;
-        for (size_t k = 0; k < plain_index.size(); ++k)
-            max_cols = std::max({max_cols, seqs[plain_pairs[2 * k]].last_real, seqs[plain_pairs[2 * k + 1]].last_real});
-        plan.plain_grid = plain_grid(plan.n_plain, dev.compute_units);
-        plan.plain_line_stride = (max_cols + 63) & ~63;
-        MSA_TRY(plan.alloc(plan.d_plain_lines, (size_t)plan.plain_grid * plan.plain_line_stride));
+        for (size_t k = 0; k < hp.plain_index.size(); ++k)
+            max_cols = std::max({max_cols, seqs[hp.plain_pairs[2 * k]].last_real,
+                                 seqs[hp.plain_pairs[2 * k + 1]].last_real});
+        hp.plain_grid = plain_grid((int64_t)hp.plain_index.size(), dev.compute_units);
+        hp.plain_stride = (max_cols + 63) & ~63;
     }
 
     // Batched part: group by (query, transpose), sort each query's targets by
-    // length, cut them into items of up to kItemTargets, and queue the items
-    // longest first.
+    // length, cut into items of up to 16 targets (four rounds of four), queue
+    // the items longest first.
     if (!fast.empty()) {
         std::sort(fast.begin(), fast.end(), [&](const Fast& x, const Fast& y) {
             if (x.query != y.query) return x.query < y.query;
@@ -328,27 +201,27 @@ int build_plan(msa_nw_plan_opaque& plan, const uint16_t* blocks, const int64_t* 
             return x.index < y.index;
         });
         constexpr int kItemTargets = 16;
-        const int strip_rows = batched_strip_rows();
         struct Item { int4 v; int64_t cost; };
         std::vector<Item> items;
-        std::vector<int2> targets(fast.size());
+        hp.targets.resize(fast.size());
         int32_t max_t = 1;
         for (size_t k = 0; k < fast.size(); ++k) {
-            targets[k] = make_int2(fast[k].target, fast[k].index);
+            hp.targets[k] = make_int2(fast[k].target, fast[k].index);
             max_t = std::max(max_t, seqs[fast[k].target].first_end);
         }
         size_t k = 0;
+        int32_t max_q = 0;
         while (k < fast.size()) {
             size_t e = k;
             while (e < fast.size() && fast[e].query == fast[k].query) ++e;
             const int32_t q = fast[k].query >> 1;
             const int32_t flags = fast[k].query & 1;
-            const int64_t lq = seqs[q].first_end;
-            const int64_t rows = ((lq + 127) / 128) * 128;
+            max_q = std::max(max_q, seqs[q].first_end);
+            const int64_t rows = ((int64_t)seqs[q].first_end + 127) / 128 * 128;
             for (size_t s = k; s < e; s += kItemTargets) {
                 const int32_t cnt = (int32_t)std::min<size_t>(kItemTargets, e - s);
                 int64_t cost = 0;
-                // A round of four targets costs as much as its longest.
+                // A round of four targets costs as much as its longest (the first).
                 for (size_t u = s; u < s + cnt; u += 4)
                     cost += rows * (((seqs[fast[u].target].first_end + 30) / 16) * 16);
                 items.push_back({make_int4(q, (int)s, cnt, flags), cost});
@@ -356,26 +229,20 @@ int build_plan(msa_nw_plan_opaque& plan, const uint16_t* blocks, const int64_t* 
             k = e;
         }
         std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.cost > y.cost; });
-        std::vector<int4> hitems(items.size());
-        for (size_t u = 0; u < items.size(); ++u) hitems[u] = items[u].v;
-        Batch& b = plan.batch;
-        b.nitems = (int32_t)items.size();
+        hp.items.resize(items.size());
+        for (size_t u = 0; u < items.size(); ++u) hp.items[u] = items[u].v;
         int per_cu = 0;
         MSA_TRY(batched_occupancy(&per_cu));
         if (per_cu < 1) return fail(MSA_ERR_RUNTIME, "batched kernel does not fit a compute unit");
-        b.grid = (int)std::min<int64_t>((int64_t)per_cu * dev.compute_units, (b.nitems + 3) / 4);
-        b.line_stride = (max_t + 63) & ~63;
-        (void)strip_rows;
-        MSA_TRY(plan.upload(b.d_items, hitems.data(), hitems.size(), stream));
-        MSA_TRY(plan.upload(b.d_targets, targets.data(), targets.size(), stream));
-        MSA_TRY(plan.alloc(b.d_lines, (size_t)b.grid * batched_groups_per_block() * b.line_stride));
-        MSA_TRY(plan.alloc(b.d_queue, 1));
-        plan.info.bins = 1;
+        hp.batch_grid = (int)std::min<int64_t>((int64_t)per_cu * dev.compute_units, ((int64_t)items.size() + 3) / 4);
+        // Lines are touched only by queries of more than one strip.
+        hp.batch_stride = max_q > batched_strip_rows() ? ((max_t + 63) & ~63) : 0;
+        hp.info.bins = 1;
     }
+
     // Long part: compact alphabet of the codes these pairs use, strip height
     // from the profile size, one pair per workgroup queued longest first.
     if (!longs.empty()) {
-        LongPart& L = plan.lng;
         bool present[32] = {};
         int32_t max_t = 1, max_q = 1;
         for (const auto& f : longs) {
@@ -388,45 +255,143 @@ int build_plan(msa_nw_plan_opaque& plan, const uint16_t* blocks, const int64_t* 
             max_q = std::max(max_q, seqs[f.query >> 1].first_end);
         }
         int slots = 0;
-        for (int code = 0; code < 32; ++code) L.slot_code[code] = 0xFF;
+        for (int code = 0; code < 32; ++code) hp.slot_code[code] = 0xFF;
         for (int code = 0; code < kEndCode; ++code)
-            if (present[code]) { L.remap[code] = (uint8_t)slots; L.slot_code[slots] = (uint8_t)code; ++slots; }
-        L.slots = slots + 1;                     // the neutral slot is last
+            if (present[code]) {
+                hp.remap[code] = (uint8_t)slots;
+                hp.slot_code[slots] = (uint8_t)code;
+                ++slots;
+            }
+        hp.long_slots = slots + 1;     // the neutral slot is last
         for (int code = 0; code < 32; ++code)
-            if (code >= kEndCode || !present[code]) L.remap[code] = (uint8_t)slots;
-        L.rows = long_rows_for_slots(L.slots);
-        const int strip = long_strip_rows(L.rows);
-        L.max_boundaries = std::max(1, (max_q + strip - 1) / strip - 1);
-        L.line_stride = (max_t + 63) & ~63;
-        L.lds = long_lds_bytes(L.rows, L.slots, L.max_boundaries);
-        if (L.lds > dev.lds_per_block)
-            return fail(MSA_ERR_TOO_LARGE, "long pairs need %zu bytes of LDS per workgroup", L.lds);
+            if (code >= kEndCode || !present[code]) hp.remap[code] = (uint8_t)slots;
+        hp.long_rows = long_rows_for_slots(hp.long_slots);
+        const int strip = long_strip_rows(hp.long_rows);
+        hp.long_boundaries = std::max(1, (max_q + strip - 1) / strip - 1);
+        hp.long_stride = (max_t + 63) & ~63;
+        hp.long_lds = long_lds_bytes(hp.long_rows, hp.long_slots, hp.long_boundaries);
+        if (hp.long_lds > dev.lds_per_block)
+            return fail(MSA_ERR_TOO_LARGE, "long pairs need %zu bytes of LDS per workgroup", hp.long_lds);
         int per_cu = 0;
-        MSA_TRY(long_occupancy(L.rows, L.lds, &per_cu));
+        MSA_TRY(long_occupancy(hp.long_rows, hp.long_lds, &per_cu));
         if (per_cu < 1) return fail(MSA_ERR_RUNTIME, "long kernel does not fit a compute unit");
         std::vector<std::pair<int64_t, int4>> order;
         for (const auto& f : longs)
             order.push_back({(int64_t)seqs[f.query >> 1].first_end * seqs[f.target].first_end,
                              make_int4(f.query >> 1, f.target, f.index, f.query & 1)});
         std::stable_sort(order.begin(), order.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
-        std::vector<int4> hp(order.size());
-        for (size_t u = 0; u < order.size(); ++u) hp[u] = order[u].second;
-        L.npairs = (int32_t)hp.size();
-        L.grid = (int)std::min<int64_t>((int64_t)per_cu * dev.compute_units, L.npairs);
-        MSA_TRY(plan.upload(L.d_pairs, hp.data(), hp.size(), stream));
-        MSA_TRY(plan.alloc(L.d_lines, (size_t)L.grid * L.max_boundaries * L.line_stride));
-        MSA_TRY(plan.alloc(L.d_queue, 1));
-        MSA_TRY(plan.alloc(L.d_error, 1));
-        MSA_HIP(hipMemsetAsync(L.d_error, 0, sizeof(int32_t), stream));
+        hp.long_pairs.resize(order.size());
+        for (size_t u = 0; u < order.size(); ++u) hp.long_pairs[u] = order[u].second;
+        hp.long_grid = (int)std::min<int64_t>((int64_t)per_cu * dev.compute_units, (int64_t)hp.long_pairs.size());
     }
-    plan.info.launches = (int32_t)((plan.batch.nitems ? 2 : 0) + (plan.lng.npairs ? 2 : 0) + (plan.n_plain ? 1 : 0));
-    plan.use_graph = opt.use_graph != 0;
-    MSA_HIP(hipStreamSynchronize(stream));
+    hp.info.launches = (int32_t)((hp.items.empty() ? 0 : 2) + (hp.long_pairs.empty() ? 0 : 2) +
+                                 (hp.plain_index.empty() ? 0 : 1));
     return MSA_OK;
 }
 
-}  // namespace
+DataLayout data_layout(const HostPlan& hp)
+{
+    DataLayout lay;
+    size_t at = 0;
+    auto place = [&](size_t& field, size_t bytes) { field = at; at = align256(at + std::max<size_t>(bytes, 1)); };
+    place(lay.blocks, hp.blocks.size() * sizeof(uint16_t));
+    place(lay.seqinfo, hp.seqinfo.size() * sizeof(SeqInfo));
+    place(lay.plain_pairs, hp.plain_pairs.size() * sizeof(int32_t));
+    place(lay.plain_index, hp.plain_index.size() * sizeof(int32_t));
+    place(lay.items, hp.items.size() * sizeof(int4));
+    place(lay.targets, hp.targets.size() * sizeof(int2));
+    place(lay.long_pairs, hp.long_pairs.size() * sizeof(int4));
+    lay.bytes = at;
+    return lay;
+}
+
+void pack_data(const HostPlan& hp, const DataLayout& lay, void* host_block)
+{
+    char* base = static_cast<char*>(host_block);
+    auto put = [&](size_t off, const void* src, size_t bytes) { if (bytes) std::memcpy(base + off, src, bytes); };
+    put(lay.blocks, hp.blocks.data(), hp.blocks.size() * sizeof(uint16_t));
+    put(lay.seqinfo, hp.seqinfo.data(), hp.seqinfo.size() * sizeof(SeqInfo));
+    put(lay.plain_pairs, hp.plain_pairs.data(), hp.plain_pairs.size() * sizeof(int32_t));
+    put(lay.plain_index, hp.plain_index.data(), hp.plain_index.size() * sizeof(int32_t));
+    put(lay.items, hp.items.data(), hp.items.size() * sizeof(int4));
+    put(lay.targets, hp.targets.data(), hp.targets.size() * sizeof(int2));
+    put(lay.long_pairs, hp.long_pairs.data(), hp.long_pairs.size() * sizeof(int4));
+}
+
+ScratchNeed scratch_need(const HostPlan& hp)
+{
+    ScratchNeed n;
+    auto up64 = [](size_t v) { return (v + 63) & ~size_t(63); };
+    if (!hp.plain_index.empty()) n.plain_lines = up64((size_t)hp.plain_grid * hp.plain_stride);
+    if (!hp.items.empty())
+        n.batch_lines = up64((size_t)hp.batch_grid * batched_groups_per_block() * hp.batch_stride);
+    if (!hp.long_pairs.empty()) n.long_lines = up64((size_t)hp.long_grid * hp.long_boundaries * hp.long_stride);
+    return n;
+}
+
+Scratch carve_scratch(const ScratchNeed& need, int32_t* base)
+{
+    Scratch s;
+    s.batch_queue = base;
+    s.long_queue = base + 64;
+    s.long_error = base + 128;
+    int32_t* at = base + 192;
+    s.plain_lines = at;
+    at += need.plain_lines;
+    s.batch_lines = at;
+    at += need.batch_lines;
+    s.long_lines = at;
+    return s;
+}
+
+int launch_plan(const HostPlan& hp, const DataLayout& lay, const void* d_data, const Scratch& scr,
+                int32_t* d_out, hipStream_t stream)
+{
+    const char* base = static_cast<const char*>(d_data);
+    const uint16_t* d_blocks = reinterpret_cast<const uint16_t*>(base + lay.blocks);
+    const SeqInfo* d_info = reinterpret_cast<const SeqInfo*>(base + lay.seqinfo);
+    if (!hp.items.empty())
+        MSA_TRY(batched_launch(hp.batch_grid, d_blocks, d_info, reinterpret_cast<const int4*>(base + lay.items),
+                               (int32_t)hp.items.size(), reinterpret_cast<const int2*>(base + lay.targets),
+                               scr.batch_queue, d_out, scr.batch_lines, std::max(hp.batch_stride, 1), hp.sxx,
+                               hp.penalty, stream));
+    if (!hp.long_pairs.empty())
+        MSA_TRY(long_launch(hp.long_rows, hp.long_grid, hp.long_lds, d_blocks, d_info,
+                            reinterpret_cast<const int4*>(base + lay.long_pairs), (int32_t)hp.long_pairs.size(),
+                            scr.long_queue, scr.long_error, d_out, scr.long_lines, hp.long_stride,
+                            hp.long_boundaries, hp.long_slots, hp.remap, hp.slot_code, hp.sxx, hp.penalty, stream));
+    if (!hp.plain_index.empty())
+        MSA_TRY(plain_launch(d_blocks, d_info, reinterpret_cast<const int32_t*>(base + lay.plain_pairs),
+                             reinterpret_cast<const int32_t*>(base + lay.plain_index),
+                             (int64_t)hp.plain_index.size(), hp.sxx, hp.penalty, d_out, scr.plain_lines,
+                             std::max(hp.plain_stride, 1), hp.plain_grid, stream));
+    return MSA_OK;
+}
+
 }  // namespace msa
+
+struct msa_nw_plan_opaque {
+    msa::HostPlan hp;
+    msa::DataLayout lay;
+    msa::ScratchNeed need;
+    msa::Scratch scratch;
+    void* d_data = nullptr;
+    int32_t* d_scratch = nullptr;
+    bool use_graph = false;
+    hipStream_t capture_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int32_t* graph_out = nullptr;
+
+    ~msa_nw_plan_opaque()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (capture_stream) (void)hipStreamDestroy(capture_stream);
+        if (d_data) (void)hipFree(d_data);
+        if (d_scratch) (void)hipFree(d_scratch);
+    }
+};
 
 using msa::fail;
 
@@ -438,15 +403,29 @@ int msa_nw_plan_create(msa_nw_plan* plan, const uint16_t* blocks, const int64_t*
 {
     if (!plan) return fail(MSA_ERR_INVALID_ARGUMENT, "plan is null");
     *plan = nullptr;
-    if (nseq > 0 && !blocks) return fail(MSA_ERR_INVALID_ARGUMENT, "blocks is null");
     msa_nw_options opt{};
     if (options) opt = *options;
     auto* p = new (std::nothrow) msa_nw_plan_opaque();
     if (!p) return fail(MSA_ERR_OUT_OF_MEMORY, "host allocation failed");
     int rc = MSA_OK;
     try {
-        rc = msa::build_plan(*p, blocks, offset, size, nseq, pairs, npairs, table625, penalty, opt,
-                             (hipStream_t)stream);
+        rc = msa::plan_host(p->hp, blocks, offset, size, nseq, pairs, npairs, table625, penalty, opt);
+        if (rc == MSA_OK) {
+            p->lay = msa::data_layout(p->hp);
+            p->need = msa::scratch_need(p->hp);
+            std::vector<char> staging(p->lay.bytes);
+            msa::pack_data(p->hp, p->lay, staging.data());
+            rc = msa::hip_status(hipMalloc(&p->d_data, p->lay.bytes), "hipMalloc");
+            if (rc == MSA_OK)
+                rc = msa::hip_status(hipMalloc(&p->d_scratch, p->need.words() * sizeof(int32_t)), "hipMalloc");
+            if (rc == MSA_OK)
+                rc = msa::hip_status(hipMemcpyAsync(p->d_data, staging.data(), p->lay.bytes, hipMemcpyHostToDevice,
+                                                    (hipStream_t)stream), "hipMemcpyAsync");
+            if (rc == MSA_OK) rc = msa::hip_status(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+            p->scratch = msa::carve_scratch(p->need, p->d_scratch);
+            p->hp.info.device_bytes = (int64_t)(p->lay.bytes + p->need.words() * sizeof(int32_t));
+            p->use_graph = opt.use_graph != 0;
+        }
     } catch (const std::bad_alloc&) {
         rc = fail(MSA_ERR_OUT_OF_MEMORY, "host allocation failed while planning");
     }
@@ -460,16 +439,16 @@ int msa_nw_plan_create(msa_nw_plan* plan, const uint16_t* blocks, const int64_t*
 
 int msa_nw_plan_execute(msa_nw_plan plan, int32_t* d_out, msa_stream stream)
 {
-    if (!plan || (!d_out && plan->info.pairs)) return fail(MSA_ERR_INVALID_ARGUMENT, "null plan or output");
-    if (!plan->info.pairs) return MSA_OK;
+    if (!plan || (!d_out && plan->hp.info.pairs)) return fail(MSA_ERR_INVALID_ARGUMENT, "null plan or output");
+    if (!plan->hp.info.pairs) return MSA_OK;
     const hipStream_t s = (hipStream_t)stream;
-    if (!plan->use_graph) return plan->launch_all(d_out, s);
+    if (!plan->use_graph) return msa::launch_plan(plan->hp, plan->lay, plan->d_data, plan->scratch, d_out, s);
     if (plan->graph_out != d_out || !plan->exec) {
         if (plan->exec) { (void)hipGraphExecDestroy(plan->exec); plan->exec = nullptr; }
         if (plan->graph) { (void)hipGraphDestroy(plan->graph); plan->graph = nullptr; }
         if (!plan->capture_stream) MSA_HIP(hipStreamCreateWithFlags(&plan->capture_stream, hipStreamNonBlocking));
         MSA_HIP(hipStreamBeginCapture(plan->capture_stream, hipStreamCaptureModeThreadLocal));
-        const int rc = plan->launch_all(d_out, plan->capture_stream);
+        const int rc = msa::launch_plan(plan->hp, plan->lay, plan->d_data, plan->scratch, d_out, plan->capture_stream);
         hipGraph_t g = nullptr;
         const hipError_t end = hipStreamEndCapture(plan->capture_stream, &g);
         if (rc != MSA_OK) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -484,7 +463,7 @@ int msa_nw_plan_execute(msa_nw_plan plan, int32_t* d_out, msa_stream stream)
 int msa_nw_plan_info_get(msa_nw_plan plan, msa_nw_plan_info* info)
 {
     if (!plan || !info) return fail(MSA_ERR_INVALID_ARGUMENT, "null plan or info");
-    *info = plan->info;
+    *info = plan->hp.info;
     return MSA_OK;
 }
 
@@ -492,9 +471,9 @@ int msa_nw_plan_status(msa_nw_plan plan, msa_stream stream)
 {
     if (!plan) return fail(MSA_ERR_INVALID_ARGUMENT, "plan is null");
     MSA_HIP(hipStreamSynchronize((hipStream_t)stream));
-    if (plan->lng.d_error) {
+    if (!plan->hp.long_pairs.empty()) {
         int32_t err = 0;
-        MSA_HIP(hipMemcpy(&err, plan->lng.d_error, sizeof(err), hipMemcpyDeviceToHost));
+        MSA_HIP(hipMemcpy(&err, plan->scratch.long_error, sizeof(err), hipMemcpyDeviceToHost));
         if (err) return fail(MSA_ERR_RUNTIME, "long kernel: a strip waited past its limit for the strip above");
     }
     return MSA_OK;
@@ -520,7 +499,8 @@ int msa_nw_align(const uint16_t* blocks, const int64_t* offset, const int32_t* s
         if (rc == MSA_OK) rc = msa_nw_plan_execute(plan, d_out, nullptr);
         if (rc == MSA_OK) rc = msa_nw_plan_status(plan, nullptr);
         if (rc == MSA_OK)
-            rc = msa::hip_status(hipMemcpy(out, d_out, (size_t)npairs * sizeof(int32_t), hipMemcpyDeviceToHost), "hipMemcpy");
+            rc = msa::hip_status(hipMemcpy(out, d_out, (size_t)npairs * sizeof(int32_t), hipMemcpyDeviceToHost),
+                                 "hipMemcpy");
         if (d_out) (void)hipFree(d_out);
     }
     msa_nw_plan_destroy(plan);
@@ -574,11 +554,11 @@ extern "C" int msa_nw_plain(const uint16_t* d_blocks, const int64_t* d_offset, c
     int dev = 0;
     MSA_HIP(hipGetDevice(&dev));
     if (dev < 0 || dev >= 64) return fail(MSA_ERR_INVALID_ARGUMENT, "device %d out of range", dev);
-    msa::DeviceState st;
-    MSA_TRY(msa::device_state(st));
+    msa::DeviceShape shape;
+    MSA_TRY(msa::device_shape(shape));
     PlainWorkspace& ws = g_workspace[dev];
     std::lock_guard<std::mutex> guard(ws.lock);
-    const int grid = msa::plain_grid(npairs, st.compute_units);
+    const int grid = msa::plain_grid(npairs, shape.compute_units);
     const int32_t stride = (int32_t)(((int64_t)3 * max_blocks + 63) & ~int64_t(63));
     MSA_TRY(grow(ws.info, ws.info_cap, (size_t)nseq));
     MSA_TRY(grow(ws.lines, ws.lines_cap, (size_t)grid * std::max(stride, 1)));

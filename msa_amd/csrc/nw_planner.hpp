@@ -1,0 +1,112 @@
+// Host planning shared by the plan API (nw_plan.hip) and the serving loop
+// (nw_serve.hip).  A HostPlan is everything the host decides for one pair set:
+// the sequence store, the routing of every pair to the plain, batched or long
+// kernel, the work lists and the launch shapes.  Its arrays are laid out as
+// one contiguous data block (one host->device copy); the per-launch scratch
+// (strip-boundary lines, queue counters, the error flag) is sized separately
+// so that a server can keep it per stream and reuse it across jobs.
+#pragma once
+
+#include "msa_internal.hpp"
+
+#include <vector>
+
+namespace msa {
+
+// Kernel entry points (nw_plain.hip, nw_batched.hip, nw_long.hip).
+int plain_launch(const uint16_t* d_blocks, const SeqInfo* d_info, const int32_t* d_pairs,
+                 const int32_t* d_out_index, int64_t npairs, const Sxx& sxx, int32_t penalty,
+                 int32_t* d_out, int32_t* d_lines, int32_t line_stride, int grid, hipStream_t stream);
+int plain_grid(int64_t npairs, int compute_units);
+int seq_info_launch(const uint16_t* d_blocks, const int64_t* d_offset, const int32_t* d_size, int32_t nseq,
+                    SeqInfo* d_info, hipStream_t stream);
+int batched_threads();
+size_t batched_lds_bytes();
+int batched_groups_per_block();
+int batched_strip_rows();
+int batched_occupancy(int* blocks_per_cu);
+int batched_launch(int grid, const uint16_t* d_blocks, const SeqInfo* d_info, const int4* d_items, int32_t nitems,
+                   const int2* d_targets, int32_t* d_queue, int32_t* d_out, int32_t* d_lines, int32_t line_stride,
+                   const Sxx& sxx, int32_t penalty, hipStream_t stream);
+int long_rows_for_slots(int slots);
+int long_strip_rows(int rows);
+size_t long_lds_bytes(int rows, int slots, int max_boundaries);
+int long_occupancy(int rows, size_t lds, int* blocks_per_cu);
+int long_launch(int rows, int grid, size_t lds, const uint16_t* d_blocks, const SeqInfo* d_info, const int4* d_pairs,
+                int32_t npairs, int32_t* d_queue, int32_t* d_error, int32_t* d_out, int32_t* d_lines,
+                int32_t line_stride, int32_t max_boundaries, int slots, const uint8_t* remap,
+                const uint8_t* slot_code, const Sxx& sxx, int32_t penalty, hipStream_t stream);
+
+constexpr int kDefaultLongThreshold = 8192;
+
+struct DeviceShape {
+    int compute_units = 0;
+    size_t lds_per_block = 0;
+};
+// Properties of the current device (cached per device).
+int device_shape(DeviceShape& out);
+
+int make_sxx(const int32_t* table625, int32_t penalty, Sxx& sxx, bool& symmetric);
+
+struct HostPlan {
+    msa_nw_plan_info info{};
+    Sxx sxx;
+    int32_t penalty = 0;
+    // sequence store
+    std::vector<uint16_t> blocks;
+    std::vector<SeqInfo> seqinfo;
+    // plain kernel
+    std::vector<int32_t> plain_pairs;     // 2 per pair
+    std::vector<int32_t> plain_index;     // output index per pair
+    int plain_grid = 0;
+    int32_t plain_stride = 0;             // ints per boundary line
+    // batched kernel
+    std::vector<int4> items;
+    std::vector<int2> targets;
+    int batch_grid = 0;
+    int32_t batch_stride = 0;
+    // long kernel
+    std::vector<int4> long_pairs;
+    int long_rows = 32, long_slots = 0, long_grid = 0;
+    size_t long_lds = 0;
+    int32_t long_stride = 0, long_boundaries = 0;
+    uint8_t remap[32] = {};
+    uint8_t slot_code[32] = {};
+};
+
+// Builds a HostPlan for the current device.  `pairs` may be null (all pairs
+// in reference order).  Input errors are reported before any device call.
+int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const int32_t* size, int32_t nseq,
+              const int32_t* pairs, int64_t npairs, const int32_t* table625, int32_t penalty,
+              const msa_nw_options& opt);
+
+// The HostPlan's arrays inside one block of memory (offsets 256-aligned).
+struct DataLayout {
+    size_t blocks = 0, seqinfo = 0, plain_pairs = 0, plain_index = 0, items = 0, targets = 0, long_pairs = 0;
+    size_t bytes = 0;
+};
+DataLayout data_layout(const HostPlan& hp);
+void pack_data(const HostPlan& hp, const DataLayout& lay, void* host_block);
+
+// Scratch one execution needs (int32 words), laid out the same way.
+struct ScratchNeed {
+    size_t plain_lines = 0, batch_lines = 0, long_lines = 0;
+    size_t words() const { return plain_lines + batch_lines + long_lines + 3 * 64; }
+};
+ScratchNeed scratch_need(const HostPlan& hp);
+
+struct Scratch {
+    int32_t* plain_lines = nullptr;
+    int32_t* batch_lines = nullptr;
+    int32_t* long_lines = nullptr;
+    int32_t* batch_queue = nullptr;
+    int32_t* long_queue = nullptr;
+    int32_t* long_error = nullptr;
+};
+Scratch carve_scratch(const ScratchNeed& need, int32_t* base);
+
+// Launches every kernel of one execution of `hp` (data at `d_data`).
+int launch_plan(const HostPlan& hp, const DataLayout& lay, const void* d_data, const Scratch& scratch,
+                int32_t* d_out, hipStream_t stream);
+
+}  // namespace msa

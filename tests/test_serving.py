@@ -1,0 +1,113 @@
+"""The serving loop (msa_srv_*): jobs admitted as they arrive, overlapping on
+the device, each job's scores identical alone and in the mix."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from msa_amd import _native as N
+from msa_amd import workloads as W
+from tests import helpers as H
+
+
+def make_jobs(rng, count, lo_n=4, hi_n=40, lo_len=20, hi_len=700, dirty_every=5):
+    jobs = []
+    for j in range(count):
+        n = int(rng.integers(lo_n, hi_n + 1))
+        jobs.append(H.random_db(rng, n, lo_len, hi_len, dirty=(j % dirty_every == 0)))
+    return jobs
+
+
+def test_server_needs_a_gpu_when_none_is_present():
+    if N.device_count() > 0:
+        pytest.skip("a GPU is present")
+    T, p = H.tables()["blosum62"]
+    with pytest.raises(N.MsaError):
+        N.Server(T, p)
+
+
+@pytest.mark.gpu
+def test_jobs_alone_and_in_the_mix():
+    rng = np.random.default_rng(21)
+    T, p = H.tables()["blosum62"]
+    jobs = make_jobs(rng, 14)
+    alone = [N.align(*db, T, p) for db in jobs]
+    for k in (0, 3):
+        assert np.array_equal(alone[k].astype(np.float32), H.oracle_scores(jobs[k], T, p))
+    srv = N.Server(T, p, slots=4, max_length=1024)
+    try:
+        ids = [srv.submit(*db) for db in jobs]          # a burst: the mix
+        results = []
+        for jid in ids:
+            srv.wait(jid, 60000)
+            results.append(srv.fetch(jid))
+        for (scores, st), ref in zip(results, alone):
+            assert np.array_equal(scores, ref)
+            assert st["arrival_us"] <= st["admitted_us"] <= st["finished_us"]
+        # Some jobs ran on the device at the same time.
+        spans = sorted((st["admitted_us"], st["finished_us"]) for _, st in results)
+        overlaps = sum(1 for a, b in zip(spans, spans[1:]) if b[0] < a[1])
+        assert overlaps >= 1
+        assert srv.stats()["completed"] == len(jobs)
+    finally:
+        srv.close()
+
+
+@pytest.mark.gpu
+def test_admission_waits_for_arena_space():
+    """An arena that holds about one job at a time still serves every job, in
+    arrival order; a job larger than the arena is refused at submission."""
+    rng = np.random.default_rng(22)
+    T, p = H.tables()["pam250"]
+    jobs = make_jobs(rng, 6, lo_n=30, hi_n=40, lo_len=200, hi_len=400, dirty_every=100)
+    alone = [N.align(*db, T, p) for db in jobs]
+    srv = N.Server(T, p, slots=4, max_length=512, device_bytes=80 << 10, host_bytes=1 << 20)
+    try:
+        ids = [srv.submit(*db) for db in jobs]
+        stats = []
+        for jid, ref in zip(ids, alone):
+            srv.wait(jid, 60000)
+            scores, st = srv.fetch(jid)
+            assert np.array_equal(scores, ref)
+            stats.append(st)
+        admitted = [st["admitted_us"] for st in stats]
+        assert admitted == sorted(admitted)
+        big = H.random_db(rng, 120, 400, 500)
+        with pytest.raises(N.MsaError) as e:
+            srv.submit(*big)
+        assert e.value.status == N.MSA_ERR_TOO_LARGE
+    finally:
+        srv.close()
+
+
+@pytest.mark.gpu
+def test_poisson_stream_with_concurrent_submitter():
+    """Jobs submitted from another thread at Poisson times while results are
+    collected: every job completes and matches the batch entry point."""
+    tables = H.tables()
+    jobs = W.serving_jobs(tables, 10, seed=5, min_seqs=8, max_seqs=48, lo=100, hi=600)
+    T, p = tables["blosum62"]
+    alone = [N.align(j.blocks, j.offsets, j.sizes, T, p) for j in jobs]
+    arrivals = W.poisson_arrivals(200.0, len(jobs), seed=50)
+    srv = N.Server(T, p, slots=4, max_length=2048)
+    ids = [None] * len(jobs)
+    try:
+        t0 = time.time()
+
+        def submitter():
+            for k, j in enumerate(jobs):
+                delay = arrivals[k] - (time.time() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+                ids[k] = srv.submit(j.blocks, j.offsets, j.sizes)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        for k, jid in enumerate(ids):
+            srv.wait(jid, 60000)
+            scores, _ = srv.fetch(jid)
+            assert np.array_equal(scores, alone[k])
+    finally:
+        srv.close()
