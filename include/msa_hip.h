@@ -121,11 +121,12 @@ enum {
 typedef struct {
     int kernel;              /* MSA_KERNEL_* */
     int use_graph;           /* capture the launches of one execution in a hipGraph */
-    int lanes_per_pair;      /* batched: 16, 32 or 64; 0 = tuned/default */
-    int rows_per_lane;       /* batched: 8, 16 or 32; 0 = tuned/default */
-    int long_threshold;      /* pairs whose shorter side exceeds this many residues take the
-                                long kernel under MSA_KERNEL_AUTO; 0 = default */
-    const char* tuning_file; /* JSON written by the autotuner; NULL = built-in defaults */
+    int lanes_per_pair;      /* batched tile shape for every pair: 16 or 32 lanes per pair ... */
+    int rows_per_lane;       /* ... and at most 16 or 32 rows per lane (32 lanes: 16);
+                                0 and 0 = the tuning table's shape per query-length bin */
+    int long_threshold;      /* pairs whose shorter side has at least this many residues take
+                                the long kernel under MSA_KERNEL_AUTO; 0 = the tuning table's */
+    const char* tuning_file; /* tuning table written by msa_nw_tune; NULL = the built-in table */
 } msa_nw_options;
 
 typedef struct {
@@ -135,7 +136,7 @@ typedef struct {
     int64_t batched_pairs;   /* pairs routed to the batched kernel */
     int64_t long_pairs;      /* pairs routed to the long kernel */
     int32_t launches;        /* kernel launches per execution */
-    int32_t bins;            /* batched length bins */
+    int32_t bins;            /* batched tile shapes in use (query-length bins after merging) */
     int64_t device_bytes;    /* device memory the plan holds */
 } msa_nw_plan_info;
 
@@ -179,6 +180,21 @@ int msa_nw_align(const uint16_t* blocks, const int64_t* offset, const int32_t* s
                  const int32_t* pairs, int64_t npairs,
                  const int32_t* table625, int32_t penalty,
                  const msa_nw_options* options, int32_t* out);
+
+/*
+ * Autotuner.  No counterpart in the reference, whose launch shape is fixed
+ * (hybrid.cu:409-423).  On the current device it times, for each query-length
+ * bin of the all-pairs set of the given database, every batched tile shape
+ * (lanes per pair x rows per lane) and, for bins of long queries, the long
+ * kernel's pipelined strips, each on a sample of about `budget_cells` cells
+ * (0 = 4e10) made of whole queries of the bin with all their targets.  The
+ * winners form a tuning table (JSON), written to `path` when not NULL and
+ * copied into `report` when not NULL (truncated to report_bytes, always
+ * terminated); msa_nw_options.tuning_file reads it.
+ */
+int msa_nw_tune(const uint16_t* blocks, const int64_t* offset, const int32_t* size, int32_t nseq,
+                const int32_t* table625, int32_t penalty, int64_t budget_cells,
+                const char* path, char* report, size_t report_bytes);
 
 /*
  * The plain kernel on caller-owned device memory: one pair per workgroup at a

@@ -5,6 +5,7 @@ The library is loaded from the package directory (built in-tree by
 if the library is missing or fails to load, every GPU entry point raises.
 """
 import ctypes
+import json
 import os
 
 import numpy as np
@@ -106,6 +107,7 @@ SIGNATURES = {
     "msa_nw_plan_status": (ctypes.c_int, [_vp, _vp]),
     "msa_nw_align": (ctypes.c_int, [_vp, _vp, _vp, _i32, _vp, _i64, _vp, _i32, ctypes.POINTER(NwOptions), _vp]),
     "msa_nw_plain": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _vp, _i64, _vp, _i32, _vp, _vp]),
+    "msa_nw_tune": (ctypes.c_int, [_vp, _vp, _vp, _i32, _vp, _i32, _i64, ctypes.c_char_p, _vp, ctypes.c_size_t]),
     "msa_srv_open": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(SrvConfig)]),
     "msa_srv_submit": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, ctypes.POINTER(ctypes.c_uint64)]),
     "msa_srv_poll": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]),
@@ -208,8 +210,9 @@ class DeviceBuffer:
             pass
 
 
-def options(kernel=KERNEL_AUTO, use_graph=False, lanes=0, rows=0, long_threshold=0):
-    return NwOptions(kernel, 1 if use_graph else 0, lanes, rows, long_threshold, None)
+def options(kernel=KERNEL_AUTO, use_graph=False, lanes=0, rows=0, long_threshold=0, tuning_file=None):
+    path = os.fsencode(tuning_file) if tuning_file else None
+    return NwOptions(kernel, 1 if use_graph else 0, lanes, rows, long_threshold, path)
 
 
 class Plan:
@@ -274,6 +277,21 @@ def align(blocks, offsets, sizes, table625, penalty, pairs=None, opts=None):
     check(lib().msa_nw_align(_ptr(blocks), _ptr(offsets), _ptr(sizes), n, _ptr(pairs), npairs,
                              table625.ctypes.data, int(penalty), ctypes.byref(o), _ptr(out)))
     return out
+
+
+def tune(blocks, offsets, sizes, table625, penalty, budget_cells=0, path=None):
+    """msa_nw_tune: times every batched tile shape (and the long scheme for long
+    queries) per query-length bin of the database's all-pairs set on the
+    current device; returns the tuning table as a dict (also written to `path`)."""
+    blocks = np.ascontiguousarray(blocks, dtype=np.uint16)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+    table625 = np.ascontiguousarray(np.asarray(table625).reshape(625), dtype=np.int32)
+    report = ctypes.create_string_buffer(1 << 20)
+    check(lib().msa_nw_tune(_ptr(blocks), _ptr(offsets), _ptr(sizes), len(sizes), table625.ctypes.data,
+                            int(penalty), int(budget_cells), os.fsencode(path) if path else None,
+                            report, len(report)))
+    return json.loads(report.value.decode())
 
 
 class Server:

@@ -1,6 +1,6 @@
 // The batched Needleman-Wunsch kernel: one persistent launch over a queue of
 // work items, each item one "query" sequence and a run of "targets" (the
-// other sequence of each pair), all sorted by length by the planner
+// other sequence of each pair), all sorted by cost by the planner
 // (nw_plan.hip).
 //
 // Which pairs come here.  Only pairs whose score is the plain global
@@ -14,40 +14,49 @@
 //
 // Shape of the computation.  Every wave of a workgroup works alone: it takes
 // an item from an atomic counter, splits its query into row strips, and its
-// four 16-lane groups score four targets at a time.  Inside a group, lane k
-// owns R consecutive rows of the strip; the DP runs as an anti-diagonal
-// wavefront -- at step s lane k updates column s-k -- with the column residue
-// and the bottom score of each lane moving one lane down per step by DPP row
-// shifts.  The substitution scores of a lane's rows come from the strip's
-// profile in the wave's LDS region: R bytes per (column code, lane), read with
-// ds_read_b128 / b64 at a 16- or 8-byte lane stride (conflict-free), one step
-// ahead of use.  A strip's bottom row goes to a per-group line in global
-// memory for the next strip.
+// lane groups (64/G groups of G lanes) score 64/G targets at a time.  Inside a
+// group, lane k owns R consecutive rows of the strip; the DP runs as an
+// anti-diagonal wavefront -- at step s lane k updates column s-k -- with the
+// column residue and the bottom score of each lane moving one lane down per
+// step by DPP shifts.  The substitution scores of a lane's rows come from the
+// strip's profile in the wave's LDS region: R bytes per (column code, lane),
+// read as 16-, 8- and 4-byte planes at a matching lane stride (conflict-free),
+// one step ahead of use.  A strip's bottom row goes to a per-group line in
+// global memory for the next strip.
+//
+// Tile configurations.  Each item carries its tile shape, picked per query
+// length by the planner from the tuning table: G lanes per pair (16 or 32) and
+// at most RMAX rows per lane (16 or 32), so a full strip is G*RMAX rows.  The
+// first strip of a query takes R0 rows per lane, R0 a multiple of 4, so at
+// most 4*G - 1 padding rows are computed per query.
 //
 // Neutral padding.  The recurrence runs in the shifted domain G = H +
 // (i+j)*penalty, where every boundary cell is 0 and every cell is
 // max(G_diag + sxx, G_up, G_left) with sxx = T + 2*penalty >= 0.  Rows and
 // columns whose profile entries are 0 and whose neighbours are all 0 stay 0,
 // so the query is padded with such rows at the top of its first strip (the
-// last real row is always lane 15, register R-1 of the last strip) and each
-// target with such columns at its left (every group ends exactly at the end
-// of a 16-step chunk).  The inner loop therefore needs no per-step masking:
-// results are captured at chunk ends.
+// last real row is always the last lane's register R-1 of the last strip) and
+// each target with such columns at its left (every group ends exactly at the
+// end of a G-step chunk).  The inner loop therefore needs no per-step
+// masking: results are captured at chunk ends.
 #include "msa_internal.hpp"
 #include "nw_device.hpp"
+#include "nw_planner.hpp"
+
+#include <type_traits>
 
 namespace msa {
 
 namespace {
 
-constexpr int kG = 16;               // lanes per pair
-constexpr int kGroups = 64 / kG;     // pairs per wave
-constexpr int kRMax = 32;            // rows per lane in a full strip
-constexpr int kStripMax = kG * kRMax;
 constexpr int kCodes = 25;           // codes 0..23 and the neutral code 24
 constexpr int kNull = 24;
 constexpr int kWaves = 4;            // waves per workgroup (independent)
-constexpr int kProfWords = kCodes * kG * kRMax / 4;   // one wave's profile region
+constexpr int kMaxGroups = 4;        // lane groups per wave at 16 lanes
+constexpr int kRStep = 4;            // rows-per-lane granularity of first strips
+constexpr int kProfWords = kCodes * kMaxTileBytes / 4;   // one wave's profile region
+
+static_assert(kTileConfigCount == 3, "tile table and kernel dispatch must agree");
 
 struct BatchArgs {
     const uint16_t* blocks;
@@ -63,46 +72,66 @@ struct BatchArgs {
     Sxx sxx;
 };
 
-// Word offset, inside one code's block, of the first word a lane reads.  Full
-// 16-byte planes come first ([plane][lane][4 words]), then, when R is not a
-// multiple of 16, an 8-byte plane ([lane][2 words]).
-template <int R>
+// Calls f(integral_constant<int, U>) for U = B..E-1, so that U can name DPP
+// controls, which must be compile-time constants.
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B> {});
+        static_for<B + 1, E>(f);
+    }
+}
+
+// Every lane of each 16-lane row reads lane U of its row.
+template <int U>
+__device__ __forceinline__ int row_bcast(int v)
+{
+    return __builtin_amdgcn_mov_dpp(v, dev::kRowNewBcast0 + U, 0xF, 0xF, false);
+}
+
+// Profile layout inside one code's block of G*R bytes: the lanes' 16-byte
+// planes first ([plane][lane][4 words]), then an 8-byte plane ([lane][2
+// words]) when R % 16 >= 8, then a 4-byte plane ([lane][word]) when R % 8 == 4.
+template <int G, int R>
 __device__ __forceinline__ void load_prof(uint32_t (&w)[R / 4], const dev::lds_u32* code_base, int k)
 {
-    constexpr int FULL = R / 16;
+    constexpr int F = R / 16, H = (R % 16) / 8, Q = (R % 8) / 4;
 #pragma unroll
-    for (int j = 0; j < FULL; ++j) {
-        const dev::u32x4 v = *(const dev::lds_u32x4*)(code_base + j * kG * 4 + k * 4);
+    for (int j = 0; j < F; ++j) {
+        const dev::u32x4 v = *(const dev::lds_u32x4*)(code_base + j * G * 4 + k * 4);
         w[4 * j + 0] = v.x;
         w[4 * j + 1] = v.y;
         w[4 * j + 2] = v.z;
         w[4 * j + 3] = v.w;
     }
-    if constexpr (R % 16 == 8) {
-        const dev::u32x2 v = *(const dev::lds_u32x2*)(code_base + FULL * kG * 4 + k * 2);
-        w[4 * FULL + 0] = v.x;
-        w[4 * FULL + 1] = v.y;
+    if constexpr (H) {
+        const dev::u32x2 v = *(const dev::lds_u32x2*)(code_base + F * G * 4 + k * 2);
+        w[4 * F + 0] = v.x;
+        w[4 * F + 1] = v.y;
     }
+    if constexpr (Q) w[4 * F + 2 * H] = code_base[F * G * 4 + H * G * 2 + k];
 }
 
 // Word index of word `w` of lane `k` inside a code block (inverse of load_prof).
-template <int R>
+template <int G, int R>
 __device__ __forceinline__ int prof_word(int k, int w)
 {
-    const int j = w >> 2;
-    if (j < R / 16) return j * kG * 4 + k * 4 + (w & 3);
-    return (R / 16) * kG * 4 + k * 2 + (w & 3);
+    constexpr int F = R / 16, H = (R % 16) / 8;
+    if (w < 4 * F) return (w >> 2) * G * 4 + k * 4 + (w & 3);
+    if (H && w < 4 * F + 2) return F * G * 4 + k * 2 + (w - 4 * F);
+    return F * G * 4 + H * G * 2 + k;
 }
 
-// Builds the profile of one strip: rows [row0, row0 + 16R) of the padded
+// Builds the profile of one strip: rows [row0, row0 + G*R) of the padded
 // query (rows below pad_top are neutral).
-template <int R>
-__device__ void build_profile(dev::lds_u32* prof, const dev::lds_u8* tab,
+template <int G, int R>
+__device__ __forceinline__ void build_profile(dev::lds_u32* prof, const dev::lds_u8* tab,
                               const uint16_t* __restrict__ qseq, int row0, int pad_top, bool transpose, int lane)
 {
     constexpr int W = R / 4;
-    constexpr int CODE_WORDS = kG * W;
-    for (int slot = lane; slot < kG * W; slot += 64) {
+    constexpr int CODE_WORDS = G * W;
+    for (int slot = lane; slot < G * W; slot += 64) {
         const int k = slot / W;
         const int w = slot - k * W;
         int codes[4];
@@ -112,7 +141,7 @@ __device__ void build_profile(dev::lds_u32* prof, const dev::lds_u8* tab,
             const int code = q >= 0 ? dev::code_at(qseq, q) : -1;
             codes[t] = code < kNull ? code : -1;
         }
-        dev::lds_u32* dst = prof + prof_word<R>(k, w);
+        dev::lds_u32* dst = prof + prof_word<G, R>(k, w);
         for (int b = 0; b < kCodes; ++b) {
             uint32_t word = 0;
 #pragma unroll
@@ -131,101 +160,141 @@ __device__ void build_profile(dev::lds_u32* prof, const dev::lds_u8* tab,
 struct TargetState {
     const uint16_t* seq;
     int len;        // columns
-    int lpad;       // neutral columns in front, so that len + lpad + 15 is a multiple of 16
-    int chunks;     // (lpad + len + 15) / 16, 0 for an idle group
+    int lpad;       // neutral columns in front, so that len + lpad + G-1 is a multiple of G
+    int chunks;     // (lpad + len + G-1) / G, 0 for an idle group
 };
 
+template <int G>
 __device__ __forceinline__ int col_code(const TargetState& t, int chunk, int k)
 {
-    const int idx = chunk * kG + k - t.lpad;
+    const int idx = chunk * G + k - t.lpad;
     return (idx >= 0 && idx < t.len) ? min(dev::code_at(t.seq, idx), kNull) : kNull;
 }
 
+template <int G>
 __device__ __forceinline__ int line_value(const TargetState& t, const int32_t* __restrict__ line, int chunk, int k)
 {
-    const int idx = chunk * kG + k - t.lpad;
+    const int idx = chunk * G + k - t.lpad;
     return (idx >= 0 && idx < t.len) ? line[idx] : 0;
 }
 
 // One strip of R rows per lane over the whole target.  LOAD: the strip above
 // left its bottom row in `line`; STORE: this strip leaves its bottom row there.
-// Returns (on lane 15 of the group) the last row's score at the last column.
-template <int R, bool LOAD, bool STORE>
-__device__ int run_strip(const dev::lds_u32* prof, const TargetState& t, int32_t* __restrict__ line,
+// Returns (on the group's last lane) the last row's score at the last column.
+//
+// Per step, each lane needs the code of its column (lane 0 of the group: the
+// next residue of the target; lane k: lane k-1's code of the step before) and
+// the cell above its first row (lane 0: the line value; lane k: lane k-1's
+// bottom cell).  With 16-lane groups (one DPP row each) the next residue is
+// broadcast from the chunk's residue vector with row_newbcast and the row
+// shift takes it as its "old" value, two moves per stream and step.  With
+// 32-lane groups the vector is shifted down the wave instead and the carry
+// into the upper row comes from a row broadcast of lane 15.
+template <int G, int R, bool LOAD, bool STORE>
+__device__ __forceinline__ int run_strip(const dev::lds_u32* prof, const TargetState& t, int32_t* __restrict__ line,
                          int wchunks, int k)
 {
+    static_assert(G == 16 || G == 32, "lane groups of 16 or 32");
     constexpr int W = R / 4;
-    constexpr int CODE_WORDS = kG * W;
+    constexpr int CODE_WORDS = G * W;
     int h[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) h[r] = 0;
     int dtop = 0;
     int result = 0;
 
-    int cc = col_code(t, 0, k);
-    int cc_next = col_code(t, 1, k);
+    int cc = col_code<G>(t, 0, k);
+    int cc_next = col_code<G>(t, 1, k);
     int ln = 0, ln_next = 0;
     if constexpr (LOAD) {
-        ln = line_value(t, line, 0, k);
-        ln_next = line_value(t, line, 1, k);
+        ln = line_value<G>(t, line, 0, k);
+        ln_next = line_value<G>(t, line, 1, k);
     }
     int outv = 0;
 
     // b(0): lane 0 takes the first residue, the other lanes start neutral.
-    int b = dev::group_shr<kG>(cc, kNull);
+    int b = dev::group_shr<G>(cc, kNull);
     uint32_t w[W];
-    load_prof<R>(w, prof + b * CODE_WORDS, k);
+    load_prof<G, R>(w, prof + b * CODE_WORDS, k);
 
     for (int c = 0; c < wchunks; ++c) {
-#pragma unroll
-        for (int u = 0; u < kG; ++u) {
-            // Residue stream one step ahead: lane 0 takes the next residue.
-            cc = (u < kG - 1) ? dev::group_shl<kG>(cc) : cc_next;
-            const int bn = dev::group_shr<kG>(cc, b);
+        static_for<0, G>([&](auto uc) {
+            constexpr int u = decltype(uc)::value;
+            // Code stream one step ahead: lane 0 takes the next residue.
+            int bn;
+            if constexpr (G == 16) {
+                int first;
+                if constexpr (u < G - 1) first = row_bcast<u + 1>(cc);
+                else first = row_bcast<0>(cc_next);
+                bn = dev::dpp<dev::kRowShr1>(first, b);
+            } else {
+                cc = u < G - 1 ? dev::group_shl<G>(cc) : cc_next;
+                bn = dev::group_shr<G>(cc, b);
+            }
             uint32_t wn[W];
-            load_prof<R>(wn, prof + bn * CODE_WORDS, k);
+            load_prof<G, R>(wn, prof + bn * CODE_WORDS, k);
 
-            const int hup = LOAD ? dev::group_shr<kG>(ln, h[R - 1]) : dev::dpp_zero<dev::kRowShr1>(h[R - 1]);
-            if constexpr (LOAD) ln = (u < kG - 1) ? dev::group_shl<kG>(ln) : ln_next;
+            int hup;
+            if constexpr (LOAD) {
+                if constexpr (G == 16) {
+                    hup = dev::dpp<dev::kRowShr1>(row_bcast<u>(ln), h[R - 1]);
+                } else {
+                    hup = dev::group_shr<G>(ln, h[R - 1]);
+                    if constexpr (u < G - 1) ln = dev::group_shl<G>(ln);
+                }
+            } else {
+                if constexpr (G == 16) hup = dev::dpp_zero<dev::kRowShr1>(h[R - 1]);
+                else hup = dev::group_shr<G>(0, h[R - 1]);
+            }
 
             dev::cells<R>(h, hup, dtop, w);
             dtop = hup;
-            if constexpr (STORE) outv = dev::dpp<dev::kRowShl1>(h[R - 1], outv);
-
+            // Bottom cells collect in the group's last DPP row: after step u
+            // with u % 16 == 15, its lane j holds the last lane's bottom cell
+            // of step u - 15 + j.
+            if constexpr (STORE) {
+                outv = dev::dpp<dev::kRowShl1>(h[R - 1], outv);
+                if constexpr (u % 16 == 15) {
+                    const int j = k - (G - 16);
+                    const int col = c * G + (u - 15) + j - (G - 1) - t.lpad;
+                    if (j >= 0 && col >= 0 && col < t.len) line[col] = outv;
+                }
+            }
             b = bn;
 #pragma unroll
             for (int q = 0; q < W; ++q) w[q] = wn[q];
-        }
-        if constexpr (STORE) {
-            const int col = c * kG + k - (kG - 1) - t.lpad;
-            if (col >= 0 && col < t.len) line[col] = outv;
-        }
+        });
+        if constexpr (G == 16) cc = cc_next;
+        if constexpr (LOAD) ln = ln_next;
         result = (c == t.chunks - 1) ? h[R - 1] : result;
-        cc_next = col_code(t, c + 2, k);
-        if constexpr (LOAD) ln_next = line_value(t, line, c + 2, k);
+        cc_next = col_code<G>(t, c + 2, k);
+        if constexpr (LOAD) ln_next = line_value<G>(t, line, c + 2, k);
     }
     return result;
 }
 
-template <bool LOAD, bool STORE>
-__device__ int run_strip_r(int R, const dev::lds_u32* prof, const TargetState& t, int32_t* line, int wchunks, int k)
+// First strips: R0 rows per lane, R0 in kRStep..RMAX.
+template <int G, int RMAX, bool STORE, int R = kRStep>
+__device__ __forceinline__ int run_first_strip(int r0, const dev::lds_u32* prof, const TargetState& t, int32_t* line, int wchunks,
+                               int k)
 {
-    switch (R) {
-    case 8: return run_strip<8, LOAD, STORE>(prof, t, line, wchunks, k);
-    case 16: return run_strip<16, LOAD, STORE>(prof, t, line, wchunks, k);
-    case 24: return run_strip<24, LOAD, STORE>(prof, t, line, wchunks, k);
-    default: return run_strip<32, LOAD, STORE>(prof, t, line, wchunks, k);
+    if constexpr (R >= RMAX) {
+        return run_strip<G, RMAX, false, STORE>(prof, t, line, wchunks, k);
+    } else {
+        if (r0 == R) return run_strip<G, R, false, STORE>(prof, t, line, wchunks, k);
+        return run_first_strip<G, RMAX, STORE, R + kRStep>(r0, prof, t, line, wchunks, k);
     }
 }
 
-__device__ void build_profile_r(int R, dev::lds_u32* prof, const dev::lds_u8* tab, const uint16_t* qseq, int row0,
+template <int G, int RMAX, int R = kRStep>
+__device__ __forceinline__ void build_profile_r(int r, dev::lds_u32* prof, const dev::lds_u8* tab, const uint16_t* qseq, int row0,
                                 int pad_top, bool transpose, int lane)
 {
-    switch (R) {
-    case 8: build_profile<8>(prof, tab, qseq, row0, pad_top, transpose, lane); break;
-    case 16: build_profile<16>(prof, tab, qseq, row0, pad_top, transpose, lane); break;
-    case 24: build_profile<24>(prof, tab, qseq, row0, pad_top, transpose, lane); break;
-    default: build_profile<32>(prof, tab, qseq, row0, pad_top, transpose, lane); break;
+    if constexpr (R >= RMAX) {
+        build_profile<G, RMAX>(prof, tab, qseq, row0, pad_top, transpose, lane);
+    } else {
+        if (r == R) build_profile<G, R>(prof, tab, qseq, row0, pad_top, transpose, lane);
+        else build_profile_r<G, RMAX, R + kRStep>(r, prof, tab, qseq, row0, pad_top, transpose, lane);
     }
 }
 
@@ -238,77 +307,91 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
-__global__ __launch_bounds__(kWaves * 64) void nw_batched_kernel(BatchArgs a)
+// All strips of one item with tile shape (G, RMAX).
+template <int G, int RMAX>
+__device__ __forceinline__ void run_item(const BatchArgs& a, const int4 item, dev::lds_u32* prof, const dev::lds_u8* tab,
+                         int32_t* lines, int lane)
+{
+    constexpr int GROUPS = 64 / G;
+    constexpr int STRIP_MAX = G * RMAX;
+    const int g = lane / G;
+    const int k = lane % G;
+    int32_t* line = lines + (int64_t)g * a.line_stride;
+    const SeqInfo qi = a.info[item.x];
+    const int Lq = qi.z;
+    const uint16_t* qseq = a.blocks + qi.x;
+    const bool transpose = (item.w & 1) != 0;
+    // Strips: a first strip of R0 rows per lane, then full strips of RMAX.
+    const int full = (Lq - 1) / STRIP_MAX;            // strips after the first
+    const int rem = Lq - full * STRIP_MAX;            // rows of the first strip
+    const int R0 = min(RMAX, ((rem + G * kRStep - 1) / (G * kRStep)) * kRStep);
+    const int pad_top = G * R0 - rem;
+    const int nst = full + 1;
+
+    const int rounds = (item.z + GROUPS - 1) / GROUPS;
+    for (int rd = 0; rd < rounds; ++rd) {
+        const int ti = rd * GROUPS + g;
+        TargetState t {a.blocks, 0, 0, 0};
+        int2 tg = make_int2(0, -1);
+        if (ti < item.z) {
+            tg = a.targets[item.y + ti];
+            const SeqInfo si = a.info[tg.x];
+            t.seq = a.blocks + si.x;
+            t.len = si.z;
+            t.chunks = (t.len + 2 * G - 2) / G;
+            t.lpad = t.chunks * G - (t.len + G - 1);
+        }
+        int wchunks = t.chunks;
+        if constexpr (GROUPS > 2) wchunks = max(wchunks, __shfl_xor(wchunks, 16));
+        if constexpr (GROUPS > 1) wchunks = max(wchunks, __shfl_xor(wchunks, 32));
+
+        int res = 0;
+        for (int st = 0; st < nst; ++st) {
+            const int R = st == 0 ? R0 : RMAX;
+            const int row0 = st == 0 ? 0 : G * R0 + (st - 1) * STRIP_MAX;
+            if (nst > 1 || rd == 0) {
+                wave_lds_sync();
+                build_profile_r<G, RMAX>(R, prof, tab, qseq, row0, pad_top, transpose, lane);
+                wave_lds_sync();
+            }
+            const bool store = st + 1 < nst;
+            if (st == 0) {
+                res = store ? run_first_strip<G, RMAX, true>(R0, prof, t, line, wchunks, k)
+                            : run_first_strip<G, RMAX, false>(R0, prof, t, line, wchunks, k);
+            } else {
+                res = store ? run_strip<G, RMAX, true, true>(prof, t, line, wchunks, k)
+                            : run_strip<G, RMAX, true, false>(prof, t, line, wchunks, k);
+            }
+            // The next strip reads this strip's bottom row from `line`.
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        }
+        if (tg.y >= 0 && k == G - 1) a.out[tg.y] = res - (Lq + t.len) * a.penalty;
+    }
+}
+
+__global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(3))) void nw_batched_kernel(BatchArgs a)
 {
     extern __shared__ uint32_t lds[];
     const int wave = threadIdx.x / 64;
     const int lane = threadIdx.x % 64;
-    const int g = lane / kG;
-    const int k = lane % kG;
     dev::lds_u32* lds_base = (dev::lds_u32*)lds;   // address-space cast: 32-bit LDS offsets
     dev::lds_u32* prof = lds_base + wave * kProfWords;
     dev::lds_u8* tab = (dev::lds_u8*)(lds_base + kWaves * kProfWords);
     for (int i = threadIdx.x; i < 640; i += kWaves * 64) tab[i] = a.sxx.v[i];
     __syncthreads();
 
-    int32_t* line = a.lines + ((int64_t)(blockIdx.x * kWaves + wave) * kGroups + g) * a.line_stride;
-    const int p = a.penalty;
+    int32_t* lines = a.lines + (int64_t)(blockIdx.x * kWaves + wave) * kMaxGroups * a.line_stride;
 
     while (true) {
         int it = 0;
         if (lane == 0) it = atomicAdd(a.queue, 1);
         it = __shfl(it, 0);
         if (it >= a.nitems) break;
-
         const int4 item = a.items[it];
-        const SeqInfo qi = a.info[item.x];
-        const int Lq = qi.z;
-        const uint16_t* qseq = a.blocks + qi.x;
-        const bool transpose = (item.w & 1) != 0;
-        // Strips: a first strip of R0 rows per lane (R0 in 8..32, step 8),
-        // then full strips of 32 rows per lane.
-        const int full = (Lq - 1) / kStripMax;            // strips after the first
-        const int rem = Lq - full * kStripMax;              // rows of the first strip (1..512)
-        const int R0 = min(kRMax, ((rem + kG * 8 - 1) / (kG * 8)) * 8);
-        const int pad_top = kG * R0 - rem;
-        const int nst = full + 1;
-
-        const int rounds = (item.z + kGroups - 1) / kGroups;
-        for (int rd = 0; rd < rounds; ++rd) {
-            const int ti = rd * kGroups + g;
-            TargetState t{a.blocks, 0, 0, 0};
-            int2 tg = make_int2(0, -1);
-            if (ti < item.z) {
-                tg = a.targets[item.y + ti];
-                const SeqInfo si = a.info[tg.x];
-                t.seq = a.blocks + si.x;
-                t.len = si.z;
-                t.chunks = (t.len + 2 * kG - 2) / kG;
-                t.lpad = t.chunks * kG - (t.len + kG - 1);
-            }
-            int wchunks = t.chunks;
-            wchunks = max(wchunks, __shfl_xor(wchunks, 16));
-            wchunks = max(wchunks, __shfl_xor(wchunks, 32));
-
-            int res = 0;
-            for (int st = 0; st < nst; ++st) {
-                const int R = st == 0 ? R0 : kRMax;
-                const int row0 = st == 0 ? 0 : kG * R0 + (st - 1) * kStripMax;
-                if (nst > 1 || rd == 0) {
-                    wave_lds_sync();
-                    build_profile_r(R, prof, tab, qseq, row0, pad_top, transpose, lane);
-                    wave_lds_sync();
-                }
-                const bool load = st > 0;
-                const bool store = st + 1 < nst;
-                if (!load && !store) res = run_strip_r<false, false>(R, prof, t, line, wchunks, k);
-                else if (!load) res = run_strip_r<false, true>(R, prof, t, line, wchunks, k);
-                else if (store) res = run_strip<kRMax, true, true>(prof, t, line, wchunks, k);
-                else res = run_strip<kRMax, true, false>(prof, t, line, wchunks, k);
-                // The next strip reads this strip's bottom row from `line`.
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-            }
-            if (tg.y >= 0 && k == kG - 1) a.out[tg.y] = res - (Lq + t.len) * p;
+        switch ((item.w >> 1) & 3) {
+        case 0: run_item<kTileConfig[0].lanes, kTileConfig[0].rows>(a, item, prof, tab, lines, lane); break;
+        case 1: run_item<kTileConfig[1].lanes, kTileConfig[1].rows>(a, item, prof, tab, lines, lane); break;
+        default: run_item<kTileConfig[2].lanes, kTileConfig[2].rows>(a, item, prof, tab, lines, lane); break;
         }
     }
 }
@@ -324,9 +407,7 @@ int batched_threads() { return kWaves * 64; }
 
 size_t batched_lds_bytes() { return (size_t)kWaves * kProfWords * 4 + 640; }
 
-int batched_groups_per_block() { return kWaves * kGroups; }
-
-int batched_strip_rows() { return kStripMax; }
+int batched_groups_per_block() { return kWaves * kMaxGroups; }
 
 int batched_occupancy(int* blocks_per_cu)
 {

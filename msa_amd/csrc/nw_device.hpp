@@ -27,6 +27,7 @@ constexpr int kRowShr1 = 0x111;
 constexpr int kWaveShl1 = 0x130;
 constexpr int kWaveShr1 = 0x138;
 constexpr int kRowBcast15 = 0x142;
+constexpr int kRowNewBcast0 = 0x150;   // row_newbcast:n = 0x150 + n (gfx90a and later)
 
 template <int CTRL, int ROW_MASK = 0xF>
 __device__ __forceinline__ int dpp(int old, int src)

@@ -30,12 +30,9 @@ namespace msa {
 
 namespace {
 
-struct HostSeq {
-    int64_t offset;
-    int32_t blocks;
-    int32_t first_end;   // rows the reference runs for it as the row sequence
-    int32_t last_real;   // columns up to its last non-padding code
-};
+size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+}  // namespace
 
 int scan_sequences(const uint16_t* blocks, const int64_t* offset, const int32_t* size, int32_t nseq,
                    std::vector<HostSeq>& out, int64_t& total_blocks)
@@ -63,10 +60,6 @@ int scan_sequences(const uint16_t* blocks, const int64_t* offset, const int32_t*
                     (long long)total_blocks);
     return MSA_OK;
 }
-
-size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
-
-}  // namespace
 
 int make_sxx(const int32_t* table625, int32_t penalty, Sxx& sxx, bool& symmetric)
 {
@@ -117,6 +110,22 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
     const int kernel = opt.kernel;
     if (kernel < MSA_KERNEL_AUTO || kernel > MSA_KERNEL_LONG)
         return fail(MSA_ERR_INVALID_ARGUMENT, "unknown kernel %d", kernel);
+    // Tile shape: forced by the options, else per query-length bin from the
+    // tuning table.
+    int forced = -1;
+    if (opt.lanes_per_pair || opt.rows_per_lane) {
+        const int lanes = opt.lanes_per_pair ? opt.lanes_per_pair : 16;
+        const int rows = opt.rows_per_lane ? opt.rows_per_lane : (lanes == 16 ? 32 : 16);
+        forced = tile_config_index(lanes, rows);
+        if (forced < 0)
+            return fail(MSA_ERR_INVALID_ARGUMENT, "no batched tile of %d lanes x %d rows per lane", lanes, rows);
+    }
+    Tuning loaded;
+    const Tuning* tuning = &builtin_tuning();
+    if (opt.tuning_file) {
+        MSA_TRY(load_tuning(opt.tuning_file, loaded));
+        tuning = &loaded;
+    }
 
     std::vector<HostSeq> seqs;
     int64_t total_blocks = 0;
@@ -130,7 +139,7 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
     std::vector<Fast> fast, longs;
     fast.reserve(kernel == MSA_KERNEL_PLAIN ? 0 : (size_t)npairs);
     const int64_t long_threshold = kernel == MSA_KERNEL_LONG ? 0
-                                 : (opt.long_threshold > 0 ? opt.long_threshold : kDefaultLongThreshold);
+                                 : (opt.long_threshold > 0 ? opt.long_threshold : tuning->long_threshold);
     int64_t cells = 0;
     auto visit = [&](int32_t first, int32_t second, int64_t idx) -> int {
         if (first < 0 || first >= nseq || second < 0 || second >= nseq)
@@ -151,7 +160,7 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
         const int32_t q = row_query ? rowref : colref;
         const int32_t t = row_query ? colref : rowref;
         const int32_t transpose = (!symmetric && !row_query) ? 1 : 0;
-        if (std::min(r.first_end, col.last_real) >= long_threshold)
+        if (kernel != MSA_KERNEL_BATCHED && std::min(r.first_end, col.last_real) >= long_threshold)
             longs.push_back({q * 2 + transpose, t, (int32_t)idx});
         else
             fast.push_back({q * 2 + transpose, t, (int32_t)idx});
@@ -191,8 +200,8 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
     }
 
     // Batched part: group by (query, transpose), sort each query's targets by
-    // length, cut into items of up to 16 targets (four rounds of four), queue
-    // the items longest first.
+    // length, cut into items of up to 16 targets, queue the items costliest
+    // first.  Each item carries the tile shape of its query's length bin.
     if (!fast.empty()) {
         std::sort(fast.begin(), fast.end(), [&](const Fast& x, const Fast& y) {
             if (x.query != y.query) return x.query < y.query;
@@ -209,21 +218,30 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
             hp.targets[k] = make_int2(fast[k].target, fast[k].index);
             max_t = std::max(max_t, seqs[fast[k].target].first_end);
         }
+        bool need_lines = false;
+        unsigned used = 0;
         size_t k = 0;
-        int32_t max_q = 0;
         while (k < fast.size()) {
             size_t e = k;
             while (e < fast.size() && fast[e].query == fast[k].query) ++e;
             const int32_t q = fast[k].query >> 1;
-            const int32_t flags = fast[k].query & 1;
-            max_q = std::max(max_q, seqs[q].first_end);
-            const int64_t rows = ((int64_t)seqs[q].first_end + 127) / 128 * 128;
+            const int32_t lq = seqs[q].first_end;
+            const int cfg = forced >= 0 ? forced : tuning_config_for(*tuning, lq);
+            const TileConfig tc = kTileConfig[cfg];
+            const int groups = 64 / tc.lanes;
+            used |= 1u << cfg;
+            need_lines = need_lines || lq > tc.lanes * tc.rows;
+            const int32_t flags = (fast[k].query & 1) | (cfg << 1);
+            // Rows computed: the query rounded up to 4 rows per lane.
+            const int64_t rows = ((int64_t)lq + 4 * tc.lanes - 1) / (4 * tc.lanes) * (4 * tc.lanes);
             for (size_t s = k; s < e; s += kItemTargets) {
                 const int32_t cnt = (int32_t)std::min<size_t>(kItemTargets, e - s);
                 int64_t cost = 0;
-                // A round of four targets costs as much as its longest (the first).
-                for (size_t u = s; u < s + cnt; u += 4)
-                    cost += rows * (((seqs[fast[u].target].first_end + 30) / 16) * 16);
+                // A round of `groups` targets costs as much as its longest (the first).
+                for (size_t u = s; u < s + cnt; u += groups) {
+                    const int64_t chunks = (seqs[fast[u].target].first_end + 2 * tc.lanes - 2) / tc.lanes;
+                    cost += rows * chunks * tc.lanes;
+                }
                 items.push_back({make_int4(q, (int)s, cnt, flags), cost});
             }
             k = e;
@@ -236,8 +254,8 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
         if (per_cu < 1) return fail(MSA_ERR_RUNTIME, "batched kernel does not fit a compute unit");
         hp.batch_grid = (int)std::min<int64_t>((int64_t)per_cu * dev.compute_units, ((int64_t)items.size() + 3) / 4);
         // Lines are touched only by queries of more than one strip.
-        hp.batch_stride = max_q > batched_strip_rows() ? ((max_t + 63) & ~63) : 0;
-        hp.info.bins = 1;
+        hp.batch_stride = need_lines ? ((max_t + 63) & ~63) : 0;
+        hp.info.bins = __builtin_popcount(used);
     }
 
     // Long part: compact alphabet of the codes these pairs use, strip height

@@ -20,10 +20,22 @@ int plain_launch(const uint16_t* d_blocks, const SeqInfo* d_info, const int32_t*
 int plain_grid(int64_t npairs, int compute_units);
 int seq_info_launch(const uint16_t* d_blocks, const int64_t* d_offset, const int32_t* d_size, int32_t nseq,
                     SeqInfo* d_info, hipStream_t stream);
+// Tile shapes of the batched kernel, chosen per item: `lanes` lanes score one
+// pair, each lane owning up to `rows` rows of a strip (strips of lanes*rows
+// query rows).  The index into this table travels in bits 1..2 of an item's
+// flags.
+struct TileConfig {
+    int lanes;
+    int rows;
+};
+constexpr int kTileConfigCount = 3;
+constexpr TileConfig kTileConfig[kTileConfigCount] = {{16, 32}, {16, 16}, {32, 16}};
+constexpr int kMaxTileBytes = 512;     // largest lanes*rows: one code's profile block
+int tile_config_index(int lanes, int rows);   // -1 when not in the table
+
 int batched_threads();
 size_t batched_lds_bytes();
 int batched_groups_per_block();
-int batched_strip_rows();
 int batched_occupancy(int* blocks_per_cu);
 int batched_launch(int grid, const uint16_t* d_blocks, const SeqInfo* d_info, const int4* d_items, int32_t nitems,
                    const int2* d_targets, int32_t* d_queue, int32_t* d_out, int32_t* d_lines, int32_t line_stride,
@@ -39,6 +51,22 @@ int long_launch(int rows, int grid, size_t lds, const uint16_t* d_blocks, const 
 
 constexpr int kDefaultLongThreshold = 8192;
 
+// Tuning table: the batched tile shape of each query-length bin and the query
+// length from which pairs take the long kernel.  Bin b covers query lengths
+// [bins[b].min_query, bins[b+1].min_query); bins[0].min_query is 0.
+struct TuneBin {
+    int32_t min_query;
+    int config;                 // index into kTileConfig
+};
+struct Tuning {
+    std::vector<TuneBin> bins;
+    int32_t long_threshold = kDefaultLongThreshold;
+};
+const Tuning& builtin_tuning();
+// Reads a table written by msa_nw_tune (nw_tune.hip).
+int load_tuning(const char* path, Tuning& out);
+int tuning_config_for(const Tuning& t, int32_t query_len);
+
 struct DeviceShape {
     int compute_units = 0;
     size_t lds_per_block = 0;
@@ -47,6 +75,16 @@ struct DeviceShape {
 int device_shape(DeviceShape& out);
 
 int make_sxx(const int32_t* table625, int32_t penalty, Sxx& sxx, bool& symmetric);
+
+struct HostSeq {
+    int64_t offset;
+    int32_t blocks;
+    int32_t first_end;   // rows the reference runs for it as the row sequence
+    int32_t last_real;   // columns up to its last non-padding code
+};
+// Validates a host sequence database and reads each sequence's padding facts.
+int scan_sequences(const uint16_t* blocks, const int64_t* offset, const int32_t* size, int32_t nseq,
+                   std::vector<HostSeq>& out, int64_t& total_blocks);
 
 struct HostPlan {
     msa_nw_plan_info info{};

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 from msa_amd import _native as N
+from msa_amd import workloads as W
 from oracle import museqa_ref as M
 from tests import helpers as H
 
@@ -95,3 +96,43 @@ def test_null_arguments():
 def test_device_count_is_callable():
     n = N.device_count()
     assert n >= 0
+
+
+def _tuning_status(tmp_path, text):
+    path = tmp_path / "t.json"
+    path.write_text(text)
+    blocks, offsets, sizes = W.pack([np.array([0, 1, 2], dtype=np.uint8), np.array([3, 1], dtype=np.uint8)])
+    table = np.zeros(625, dtype=np.int32)
+    out = np.zeros(1, dtype=np.int32)
+    opts = N.options(tuning_file=str(path))
+    rc = N.lib().msa_nw_align(blocks.ctypes.data, offsets.ctypes.data, sizes.ctypes.data, 2, None, 1,
+                              table.ctypes.data, 4, ctypes.byref(opts), out.ctypes.data)
+    return rc, N.lib().msa_last_error().decode()
+
+
+@pytest.mark.parametrize("text,needle", [
+    ("not json", "not a JSON object"),
+    ('{"format": "other/1", "long_threshold": 8192, "bins": [{"min_query": 0, "lanes": 16, "rows": 32}]}',
+     "format"),
+    ('{"format": "msa-nw-tuning/1", "bins": [{"min_query": 0, "lanes": 16, "rows": 32}]}', "long_threshold"),
+    ('{"format": "msa-nw-tuning/1", "long_threshold": 8192, "bins": []}', "non-empty"),
+    ('{"format": "msa-nw-tuning/1", "long_threshold": 8192, "bins": [{"min_query": 0, "lanes": 64, "rows": 32}]}',
+     "does not have"),
+    ('{"format": "msa-nw-tuning/1", "long_threshold": 8192, "bins": [{"min_query": 5, "lanes": 16, "rows": 32}]}',
+     "start at 0"),
+    ('{"format": "msa-nw-tuning/1", "long_threshold": 8192, "bins": [{"min_query": 0, "lanes": 16, "rows": 32},'
+     ' {"min_query": 0, "lanes": 16, "rows": 16}]}', "increase"),
+])
+def test_tuning_table_errors_are_reported_before_any_device_call(tmp_path, text, needle):
+    rc, msg = _tuning_status(tmp_path, text)
+    assert rc == N.MSA_ERR_INVALID_ARGUMENT
+    assert needle in msg
+
+
+def test_valid_tuning_table_is_accepted(tmp_path):
+    """A well-formed table passes the reader: what fails next (here, with no
+    GPU) is the device, not the table."""
+    rc, msg = _tuning_status(tmp_path, '{"format": "msa-nw-tuning/1", "device": "x", "long_threshold": 4096, '
+                                       '"bins": [{"min_query": 0, "lanes": 16, "rows": 32}, '
+                                       '{"min_query": 256, "lanes": 32, "rows": 16}], "measurements": []}')
+    assert rc != N.MSA_ERR_INVALID_ARGUMENT, msg

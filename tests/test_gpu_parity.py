@@ -259,3 +259,69 @@ def test_symmetry_of_scores():
     pairs = np.array([(i, j) for i in range(25) for j in range(25)], dtype=np.int32)
     got = gpu_scores(db, T, p, pairs=pairs).reshape(25, 25)
     assert (got == got.T).all()
+
+
+TILES = [(16, 32), (16, 16), (32, 16)]
+
+
+@pytest.mark.parametrize("lanes,rows", TILES)
+@pytest.mark.parametrize("lo,hi", [(1, 140), (100, 520), (250, 1100), (900, 2100)])
+def test_batched_tile_shapes(lanes, rows, lo, hi):
+    """Every tile shape of the batched kernel, forced for all pairs: first
+    strips of every height (4..rows rows per lane) and queries of one to
+    several strips (strip boundary line in global memory)."""
+    rng = np.random.default_rng(lanes * 1000 + rows * 100 + lo + hi)
+    T, p = H.tables()["blosum62"]
+    db = H.random_db(rng, 12, lo, hi)
+    assert_oracle(db, T, p, kernel="batched", lanes=lanes, rows=rows)
+
+
+@pytest.mark.parametrize("lanes,rows", TILES)
+def test_batched_tile_shapes_asymmetric(lanes, rows):
+    """Transposed profiles under every tile shape."""
+    rng = np.random.default_rng(lanes + rows)
+    T, p = H.tables()["blosum62"]
+    Ta = T.copy()
+    Ta[0, 1], Ta[1, 0], Ta[5, 9], Ta[20, 3] = 5, -3, 4, -4
+    db = H.random_db(rng, 16, 1, 700)
+    assert_oracle(db, Ta, p, kernel="batched", lanes=lanes, rows=rows)
+
+
+def test_unknown_tile_shape_rejected():
+    T, p = H.tables()["blosum62"]
+    db = H.strings_db(["ACGT", "ACG"])
+    for lanes, rows in [(64, 16), (16, 8), (32, 32), (8, 32)]:
+        with pytest.raises(N.MsaError) as e:
+            gpu_scores(db, T, p, kernel="batched", lanes=lanes, rows=rows)
+        assert e.value.status == N.MSA_ERR_INVALID_ARGUMENT
+
+
+def test_tuner_writes_a_table_that_plans_read(tmp_path):
+    """msa_nw_tune on a small uneven database: one measurement per tile shape
+    per populated length bin (plus the long scheme from 1024 up), a table
+    that msa_nw_options.tuning_file reads, and scores that still match the
+    oracle under it."""
+    rng = np.random.default_rng(12)
+    T, p = H.tables()["blosum62"]
+    lengths = np.clip(np.exp(rng.normal(np.log(300), 0.9, 48)), 20, 2500).astype(int)
+    codes = [rng.integers(0, 20, int(n)).astype(np.uint8) for n in lengths]
+    db = W.pack(codes)
+    path = str(tmp_path / "tuning.json")
+    table = N.tune(*db, T, p, budget_cells=int(2e8), path=path)
+    assert table["format"] == "msa-nw-tuning/1"
+    assert table["bins"][0]["min_query"] == 0
+    mins = [b["min_query"] for b in table["bins"]]
+    assert mins == sorted(set(mins))
+    for b in table["bins"]:
+        assert (b["lanes"], b["rows"]) in TILES
+    shapes = {}
+    for m in table["measurements"]:
+        shapes.setdefault(m["min_query"], set()).add((m["scheme"], m["lanes"], m["rows"]))
+        assert m["gcups"] > 0 and m["ms"] > 0 and m["pairs"] > 0
+    for lo, got in shapes.items():
+        want = {("batched", a, b) for a, b in TILES} | ({("long", 64, 0)} if lo >= 1024 else set())
+        assert got == want, (lo, got)
+    assert table["long_threshold"] >= 1
+    with open(path) as f:
+        assert f.read().startswith("{")
+    assert_oracle(db, T, p, kernel="auto", tuning_file=path)
