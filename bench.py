@@ -33,6 +33,7 @@ sys.path.insert(0, ROOT)
 METRIC = "all-pairs NW GCUPS on 1 MI355X (batched vs plain, bit-exact) + serving p50/p99 latency"
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12     # int32 lane-ops per second, in T
 OPS_PER_CELL = 2
+DEFAULT_TUNING = os.path.join(ROOT, "msa_amd", "tuning", "gfx950.json")
 
 
 def log(*a):
@@ -72,11 +73,29 @@ class Timer:
         return ms.value
 
 
-def run_plan(N, w, kernel, steps, warmup, stream, out):
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload` as the committed rocprofv3
+    counter summary states them (a line `traffic <workload> <kernel> <bytes>`),
+    or None when no summary covers the pair."""
+    folder = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(folder):
+        return None
+    for name in sorted(os.listdir(folder)):
+        if not name.endswith(".md"):
+            continue
+        with open(os.path.join(folder, name)) as f:
+            for line in f:
+                parts = line.strip().strip("`").split()
+                if len(parts) == 4 and parts[0] == "traffic" and parts[1] == workload and parts[2] == kernel:
+                    return int(parts[3])
+    return None
+
+
+def run_plan(N, w, kernel, steps, warmup, stream, out, tuning=None):
     L = N.lib()
     t0 = time.time()
     plan = N.Plan(w.blocks, w.offsets, w.sizes, w.table, w.penalty,
-                  opts=N.options(kernel, use_graph=True), stream=stream)
+                  opts=N.options(kernel, use_graph=True, tuning_file=tuning), stream=stream)
     plan_s = time.time() - t0
     for _ in range(warmup):
         plan.execute(out.ptr, stream)
@@ -215,6 +234,21 @@ def serving_run(N, tables, rates, seconds, pool_size=48, slots=4, seed=5, log_fn
     return out
 
 
+def tuning_summary(path):
+    if not path:
+        return "built-in (16 lanes x 32 rows, long from 8192)"
+    with open(path) as f:
+        t = json.load(f)
+    return {"file": os.path.relpath(path, ROOT), "bins": t["bins"], "long_threshold": t["long_threshold"]}
+
+
+def dominant_kernel(info):
+    """The kernel family that scores most of the plan's pairs' cells (by pairs)."""
+    counts = {"nw_batched_kernel": info.batched_pairs, "nw_long_kernel": info.long_pairs,
+              "nw_plain_kernel": info.plain_pairs}
+    return max(counts, key=lambda k: counts[k])
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -229,6 +263,8 @@ def main():
     ap.add_argument("--rates", default="10,20,50,100,200,500")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-rate", type=float, default=100.0, help="rate of the serving figure in the main line")
+    ap.add_argument("--tuning", default=DEFAULT_TUNING,
+                    help="tuning table for the batched plan (default: the committed gfx950 table; '' = built-in)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -260,7 +296,8 @@ def main():
     stream = stream.value
     out = N.DeviceBuffer(4 * w.npairs)
 
-    batched = run_plan(N, w, N.KERNEL_AUTO, args.steps, args.warmup, stream, out)
+    tuning = args.tuning if args.tuning and os.path.exists(args.tuning) else None
+    batched = run_plan(N, w, N.KERNEL_AUTO, args.steps, args.warmup, stream, out, tuning=tuning)
     log("batched: %.2f ms/step, %.1f GCUPS" % (batched["ms"], cells / batched["ms"] / 1e6))
     plain = run_plan(N, w, N.KERNEL_PLAIN, args.plain_steps, 1, stream, out)
     log("plain:   %.2f ms/step, %.1f GCUPS" % (plain["ms"], cells / plain["ms"] / 1e6))
@@ -291,6 +328,7 @@ def main():
             "batched_pairs": int(info.batched_pairs), "long_pairs": int(info.long_pairs),
             "plain_routed_pairs": int(info.plain_pairs),
             "graph": True, "parallelism": "single GPU",
+            "tuning": tuning_summary(tuning),
         },
         "plain_gcups": round(cells / plain["ms"] / 1e6, 2),
         "plain_ms_per_step": round(plain["ms"], 3),
@@ -299,8 +337,9 @@ def main():
                       "oracle_sample_equal": oracle_ok},
         "wall_ms_per_step": round(batched["wall_ms"], 3),
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2),
-                     "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
-                     "ops_per_cell": OPS_PER_CELL},
+                     "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
+                     "traffic": measured_traffic(w.name, dominant_kernel(info)),
+                     "kernel": dominant_kernel(info), "ops_per_cell": OPS_PER_CELL},
     }
     if args.serve_rate > 0:
         result["serving"] = serving_run(N, tables, [args.serve_rate], args.serve_seconds)[0]

@@ -126,7 +126,9 @@ typedef struct {
                                 0 and 0 = the tuning table's shape per query-length bin */
     int long_threshold;      /* pairs whose shorter side has at least this many residues take
                                 the long kernel under MSA_KERNEL_AUTO; 0 = the tuning table's */
-    const char* tuning_file; /* tuning table written by msa_nw_tune; NULL = the built-in table */
+    const char* tuning_file; /* tuning table written by msa_nw_tune; NULL = the file named by the
+                                environment variable MSA_NW_TUNING when set, else the built-in
+                                table (16 lanes x 32 rows for every query, long from 8192) */
 } msa_nw_options;
 
 typedef struct {

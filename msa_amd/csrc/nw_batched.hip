@@ -371,7 +371,7 @@ __device__ __forceinline__ void run_item(const BatchArgs& a, const int4 item, de
 
 __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(3))) void nw_batched_kernel(BatchArgs a)
 {
-    extern __shared__ uint32_t lds[];
+    extern __shared__ __attribute__((aligned(16))) uint32_t lds[];   // 16-byte aligned: ds_read_b128 of profiles
     const int wave = threadIdx.x / 64;
     const int lane = threadIdx.x % 64;
     dev::lds_u32* lds_base = (dev::lds_u32*)lds;   // address-space cast: 32-bit LDS offsets

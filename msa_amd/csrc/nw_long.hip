@@ -220,7 +220,7 @@ __device__ int run_strip(const dev::lds_u32* prof, const dev::lds_u8* remap, int
 template <int R, int NW>
 __global__ __launch_bounds__(NW * 64) void nw_long_kernel(LongArgs a)
 {
-    extern __shared__ uint32_t lds[];
+    extern __shared__ __attribute__((aligned(16))) uint32_t lds[];   // 16-byte aligned: ds_read_b128 of profiles
     constexpr int STRIP = kLanes * R;
     const int wave = threadIdx.x / 64;
     const int lane = threadIdx.x % 64;

@@ -54,7 +54,7 @@ __global__ __launch_bounds__(kPlainLanes) void nw_plain_kernel(PlainArgs a)
 {
     constexpr int G = kPlainLanes;
     constexpr int W = R / 4;
-    __shared__ uint32_t prof[kPlainCodes * G * W];   // [code][lane][W words]
+    __shared__ __attribute__((aligned(16))) uint32_t prof[kPlainCodes * G * W];   // [code][lane][W words]
     __shared__ uint8_t tab[640];
 
     const int lane = threadIdx.x;

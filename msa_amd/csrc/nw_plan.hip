@@ -21,6 +21,7 @@
 #include "nw_planner.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -122,8 +123,13 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
     }
     Tuning loaded;
     const Tuning* tuning = &builtin_tuning();
-    if (opt.tuning_file) {
-        MSA_TRY(load_tuning(opt.tuning_file, loaded));
+    const char* tuning_file = opt.tuning_file;
+    if (!tuning_file) {
+        const char* env = std::getenv("MSA_NW_TUNING");
+        if (env && *env) tuning_file = env;
+    }
+    if (tuning_file) {
+        MSA_TRY(load_tuning(tuning_file, loaded));
         tuning = &loaded;
     }
 
