@@ -185,14 +185,15 @@ int msa_nw_align(const uint16_t* blocks, const int64_t* offset, const int32_t* s
 
 /*
  * Autotuner.  No counterpart in the reference, whose launch shape is fixed
- * (hybrid.cu:409-423).  On the current device it times, for each query-length
- * bin of the all-pairs set of the given database, every batched tile shape
- * (lanes per pair x rows per lane) and, for bins of long queries, the long
- * kernel's pipelined strips, each on a sample of about `budget_cells` cells
- * (0 = 4e10) made of whole queries of the bin with all their targets.  The
- * winners form a tuning table (JSON), written to `path` when not NULL and
- * copied into `report` when not NULL (truncated to report_bytes, always
- * terminated); msa_nw_options.tuning_file reads it.
+ * (hybrid.cu:409-423).  On the current device it picks, by coordinate descent
+ * over timed executions of the given database's all-pairs set (or, when that
+ * exceeds `budget_cells` cells (0 = 2e12), of every k-th query in length order
+ * with all its longer partners), a batched tile shape (lanes per pair x rows
+ * per lane) for each query-length bin and the query length from which pairs
+ * take the long kernel's pipelined strips.  The winners form a tuning table
+ * (JSON), written to `path` when not NULL and copied into `report` when not
+ * NULL (truncated to report_bytes, always terminated);
+ * msa_nw_options.tuning_file reads it.
  */
 int msa_nw_tune(const uint16_t* blocks, const int64_t* offset, const int32_t* size, int32_t nseq,
                 const int32_t* table625, int32_t penalty, int64_t budget_cells,

@@ -98,7 +98,7 @@ int device_shape(DeviceShape& out)
 
 int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const int32_t* size, int32_t nseq,
               const int32_t* pairs, int64_t npairs, const int32_t* table625, int32_t penalty,
-              const msa_nw_options& opt)
+              const msa_nw_options& opt, const Tuning* tuning_override)
 {
     if (nseq < 0 || npairs < 0) return fail(MSA_ERR_INVALID_ARGUMENT, "negative sizes");
     if (nseq > 0 && (!offset || !size || !blocks)) return fail(MSA_ERR_INVALID_ARGUMENT, "null sequence database");
@@ -123,8 +123,10 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
     }
     Tuning loaded;
     const Tuning* tuning = &builtin_tuning();
-    const char* tuning_file = opt.tuning_file;
-    if (!tuning_file) {
+    const char* tuning_file = tuning_override ? nullptr : opt.tuning_file;
+    if (tuning_override) {
+        tuning = tuning_override;
+    } else if (!tuning_file) {
         const char* env = std::getenv("MSA_NW_TUNING");
         if (env && *env) tuning_file = env;
     }
@@ -425,15 +427,25 @@ int msa_nw_plan_create(msa_nw_plan* plan, const uint16_t* blocks, const int64_t*
                        int32_t nseq, const int32_t* pairs, int64_t npairs, const int32_t* table625, int32_t penalty,
                        const msa_nw_options* options, msa_stream stream)
 {
-    if (!plan) return fail(MSA_ERR_INVALID_ARGUMENT, "plan is null");
-    *plan = nullptr;
     msa_nw_options opt{};
     if (options) opt = *options;
+    return msa::create_plan(plan, blocks, offset, size, nseq, pairs, npairs, table625, penalty, opt, nullptr,
+                            (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+int msa::create_plan(msa_nw_plan* plan, const uint16_t* blocks, const int64_t* offset, const int32_t* size,
+                     int32_t nseq, const int32_t* pairs, int64_t npairs, const int32_t* table625, int32_t penalty,
+                     const msa_nw_options& opt, const Tuning* tuning, hipStream_t stream)
+{
+    if (!plan) return fail(MSA_ERR_INVALID_ARGUMENT, "plan is null");
+    *plan = nullptr;
     auto* p = new (std::nothrow) msa_nw_plan_opaque();
     if (!p) return fail(MSA_ERR_OUT_OF_MEMORY, "host allocation failed");
     int rc = MSA_OK;
     try {
-        rc = msa::plan_host(p->hp, blocks, offset, size, nseq, pairs, npairs, table625, penalty, opt);
+        rc = msa::plan_host(p->hp, blocks, offset, size, nseq, pairs, npairs, table625, penalty, opt, tuning);
         if (rc == MSA_OK) {
             p->lay = msa::data_layout(p->hp);
             p->need = msa::scratch_need(p->hp);
@@ -444,8 +456,8 @@ int msa_nw_plan_create(msa_nw_plan* plan, const uint16_t* blocks, const int64_t*
                 rc = msa::hip_status(hipMalloc(&p->d_scratch, p->need.words() * sizeof(int32_t)), "hipMalloc");
             if (rc == MSA_OK)
                 rc = msa::hip_status(hipMemcpyAsync(p->d_data, staging.data(), p->lay.bytes, hipMemcpyHostToDevice,
-                                                    (hipStream_t)stream), "hipMemcpyAsync");
-            if (rc == MSA_OK) rc = msa::hip_status(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+                                                    stream), "hipMemcpyAsync");
+            if (rc == MSA_OK) rc = msa::hip_status(hipStreamSynchronize(stream), "hipStreamSynchronize");
             p->scratch = msa::carve_scratch(p->need, p->d_scratch);
             p->hp.info.device_bytes = (int64_t)(p->lay.bytes + p->need.words() * sizeof(int32_t));
             p->use_graph = opt.use_graph != 0;
@@ -460,6 +472,8 @@ int msa_nw_plan_create(msa_nw_plan* plan, const uint16_t* blocks, const int64_t*
     *plan = p;
     return MSA_OK;
 }
+
+extern "C" {
 
 int msa_nw_plan_execute(msa_nw_plan plan, int32_t* d_out, msa_stream stream)
 {

@@ -114,9 +114,15 @@ struct HostPlan {
 
 // Builds a HostPlan for the current device.  `pairs` may be null (all pairs
 // in reference order).  Input errors are reported before any device call.
+// `tuning`, when not null, replaces the table the options name.
 int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const int32_t* size, int32_t nseq,
               const int32_t* pairs, int64_t npairs, const int32_t* table625, int32_t penalty,
-              const msa_nw_options& opt);
+              const msa_nw_options& opt, const Tuning* tuning = nullptr);
+
+// msa_nw_plan_create with an explicit tuning table (null: from the options).
+int create_plan(msa_nw_plan* plan, const uint16_t* blocks, const int64_t* offset, const int32_t* size,
+                int32_t nseq, const int32_t* pairs, int64_t npairs, const int32_t* table625, int32_t penalty,
+                const msa_nw_options& opt, const Tuning* tuning, hipStream_t stream);
 
 // The HostPlan's arrays inside one block of memory (offsets 256-aligned).
 struct DataLayout {

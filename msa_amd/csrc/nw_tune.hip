@@ -4,13 +4,14 @@
 // (src/pairwise/needleman/impl/hybrid.cu:409-423).  Here the batched kernel
 // has several tile shapes (lanes per pair x rows per lane, kTileConfig) and a
 // second inter-tile scheme, the long kernel's pipelined strips.  Which one is
-// fastest depends on the query length, so the tuner times them on samples of
-// each query-length bin of a real database and keeps the winners:
+// fastest depends on the query length, so the tuner picks a shape per
+// query-length bin and the length from which pairs take the long kernel, by
+// timing whole executions of a real database (or a sample of its queries):
 //
 //   { "format": "msa-nw-tuning/1", "device": "...", "compute_units": 256,
 //     "long_threshold": 8192,
 //     "bins": [ {"min_query": 0, "lanes": 16, "rows": 32}, ... ],
-//     "measurements": [ {...}, ... ] }
+//     "bin_cells": [...], "threshold_trials": [...], "measurements": [...] }
 //
 // Bins run from their min_query to the next bin's; the first starts at 0.
 // Pairs whose shorter side reaches long_threshold take the long kernel.
@@ -245,48 +246,55 @@ using msa::fail;
 
 constexpr int32_t kBinEdges[] = {0, 128, 256, 512, 1024, 2048, 4096, 8192, 16384};
 constexpr int kBins = sizeof(kBinEdges) / sizeof(kBinEdges[0]);
-constexpr int64_t kMinSamplePairs = 16384;     // enough items to fill every wave of the GPU
+constexpr int32_t kLongCandidates[] = {1024, 2048, 4096, 8192, 16384, INT32_MAX};
+constexpr int kReps = 3;
+constexpr double kMinGain = 0.005;     // a change must save more than this fraction of the time
 
-struct Measurement {
-    int bin;
-    int64_t pairs, cells;
-    bool long_scheme;
-    int config;
-    double ms, gcups;
+// One timed configuration: tile shape per bin and long threshold.
+struct Trial {
+    std::string step;
+    int shape[kBins];
+    int32_t long_threshold;
+    double ms;
 };
 
-// Times one execution of a plan over `pairs` (average of enough repeats to
-// cover ~30 ms after one warm-up execution).
+msa::Tuning make_tuning(const int (&shape)[kBins], int32_t long_threshold)
+{
+    msa::Tuning t;
+    for (int b = 0; b < kBins; ++b)
+        if (b == 0 || shape[b] != shape[b - 1]) t.bins.push_back({kBinEdges[b], shape[b]});
+    t.long_threshold = long_threshold;
+    return t;
+}
+
+// Median time of kReps executions (after one warm-up) of a plan over the
+// sample under one tuning table.
 int time_plan(const uint16_t* blocks, const int64_t* offset, const int32_t* size, int32_t nseq,
-              const std::vector<int32_t>& pairs, const int32_t* table625, int32_t penalty, const msa_nw_options& opt,
-              int32_t* d_out, double& ms)
+              const std::vector<int32_t>& pairs, bool all_pairs, int64_t npairs, const int32_t* table625,
+              int32_t penalty, const msa::Tuning& tuning, int32_t* d_out, double& ms)
 {
     msa_nw_plan plan = nullptr;
-    const int64_t npairs = (int64_t)pairs.size() / 2;
-    MSA_TRY(msa_nw_plan_create(&plan, blocks, offset, size, nseq, pairs.data(), npairs, table625, penalty, &opt,
-                               nullptr));
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    int rc = msa::hip_status(hipEventCreate(&e0), "hipEventCreate");
-    if (rc == MSA_OK) rc = msa::hip_status(hipEventCreate(&e1), "hipEventCreate");
-    float t1 = 0;
+    msa_nw_options opt {};
+    opt.kernel = MSA_KERNEL_AUTO;
+    MSA_TRY(msa::create_plan(&plan, blocks, offset, size, nseq, all_pairs ? nullptr : pairs.data(), npairs, table625,
+                             penalty, opt, &tuning, nullptr));
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int rc = msa::hip_status(hipEventCreate(&ev[0]), "hipEventCreate");
+    if (rc == MSA_OK) rc = msa::hip_status(hipEventCreate(&ev[1]), "hipEventCreate");
     if (rc == MSA_OK) rc = msa_nw_plan_execute(plan, d_out, nullptr);
-    if (rc == MSA_OK) rc = msa::hip_status(hipEventRecord(e0, nullptr), "hipEventRecord");
-    if (rc == MSA_OK) rc = msa_nw_plan_execute(plan, d_out, nullptr);
-    if (rc == MSA_OK) rc = msa::hip_status(hipEventRecord(e1, nullptr), "hipEventRecord");
     if (rc == MSA_OK) rc = msa_nw_plan_status(plan, nullptr);
-    if (rc == MSA_OK) rc = msa::hip_status(hipEventElapsedTime(&t1, e0, e1), "hipEventElapsedTime");
-    int reps = 1;
-    if (rc == MSA_OK && t1 < 30.0f) {
-        reps = std::min(20, std::max(2, (int)std::ceil(30.0 / std::max(t1, 0.01f))));
-        rc = msa::hip_status(hipEventRecord(e0, nullptr), "hipEventRecord");
-        for (int r = 0; r < reps && rc == MSA_OK; ++r) rc = msa_nw_plan_execute(plan, d_out, nullptr);
-        if (rc == MSA_OK) rc = msa::hip_status(hipEventRecord(e1, nullptr), "hipEventRecord");
+    float t[kReps] = {};
+    for (int r = 0; r < kReps && rc == MSA_OK; ++r) {
+        rc = msa::hip_status(hipEventRecord(ev[0], nullptr), "hipEventRecord");
+        if (rc == MSA_OK) rc = msa_nw_plan_execute(plan, d_out, nullptr);
+        if (rc == MSA_OK) rc = msa::hip_status(hipEventRecord(ev[1], nullptr), "hipEventRecord");
         if (rc == MSA_OK) rc = msa_nw_plan_status(plan, nullptr);
-        if (rc == MSA_OK) rc = msa::hip_status(hipEventElapsedTime(&t1, e0, e1), "hipEventElapsedTime");
+        if (rc == MSA_OK) rc = msa::hip_status(hipEventElapsedTime(&t[r], ev[0], ev[1]), "hipEventElapsedTime");
     }
-    ms = (double)t1 / reps;
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
+    std::sort(t, t + kReps);
+    ms = t[kReps / 2];
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
     const int rc2 = msa_nw_plan_destroy(plan);
     return rc != MSA_OK ? rc : rc2;
 }
@@ -302,13 +310,20 @@ void appendf(std::string& s, const char* fmt, ...)
     if (n > 0) s.append(buf, std::min<size_t>((size_t)n, sizeof(buf) - 1));
 }
 
+// Coordinate descent over executions of the whole sample: each populated
+// query-length bin in turn tries every tile shape with the others fixed,
+// then every long threshold is tried with the chosen shapes.  A change is
+// kept only when it saves more than kMinGain of the time, so that timing
+// noise does not pick shapes.  Timing the
+// whole mix (not each bin alone) keeps the GPU as full as in real use, so a
+// small bin is judged by what it adds to a full launch, not by its own tail.
 int tune(const uint16_t* blocks, const int64_t* offset, const int32_t* size, int32_t nseq, const int32_t* table625,
          int32_t penalty, int64_t budget_cells, std::string& json)
 {
     if (nseq < 2) return fail(MSA_ERR_INVALID_ARGUMENT, "tuning needs at least two sequences");
-    if (nseq > 0 && (!blocks || !offset || !size)) return fail(MSA_ERR_INVALID_ARGUMENT, "null sequence database");
+    if (!blocks || !offset || !size) return fail(MSA_ERR_INVALID_ARGUMENT, "null sequence database");
     if (!table625) return fail(MSA_ERR_INVALID_ARGUMENT, "null table");
-    if (budget_cells <= 0) budget_cells = (int64_t)4e10;
+    if (budget_cells <= 0) budget_cells = (int64_t)2e12;
     std::vector<msa::HostSeq> seqs;
     int64_t total_blocks = 0;
     MSA_TRY(msa::scan_sequences(blocks, offset, size, nseq, seqs, total_blocks));
@@ -316,16 +331,42 @@ int tune(const uint16_t* blocks, const int64_t* offset, const int32_t* size, int
     bool symmetric = true;
     MSA_TRY(msa::make_sxx(table625, penalty, sxx, symmetric));
 
-    // Clean sequences in length order; a sequence is the query of the pairs
-    // it makes with every later one (the shorter side of each pair).
-    std::vector<int32_t> order;
-    for (int32_t s = 0; s < nseq; ++s)
-        if (seqs[s].first_end == seqs[s].last_real && seqs[s].first_end > 0) order.push_back(s);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int32_t a, int32_t b) { return seqs[a].first_end < seqs[b].first_end; });
-    const size_t m = order.size();
-    std::vector<int64_t> suffix_cells(m + 1, 0);   // sum of lengths after position i
-    for (size_t i = m; i-- > 0;) suffix_cells[i] = suffix_cells[i + 1] + seqs[order[i]].first_end;
+    // Query lengths: the shorter side of each pair (first padding code).
+    auto len = [&](int32_t s) { return std::min(seqs[s].first_end, seqs[s].last_real); };
+    std::vector<int32_t> order((size_t)nseq);
+    for (int32_t s = 0; s < nseq; ++s) order[(size_t)s] = s;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return len(a) < len(b); });
+    std::vector<int64_t> suffix((size_t)nseq + 1, 0);
+    for (size_t i = (size_t)nseq; i-- > 0;) suffix[i] = suffix[i + 1] + len(order[i]);
+    int64_t total_cells = 0;
+    for (size_t i = 0; i < (size_t)nseq; ++i) total_cells += (int64_t)len(order[i]) * suffix[i + 1];
+
+    // The sample: every pair, or every stride-th query (in length order)
+    // with all its longer partners.
+    const int64_t stride = std::max<int64_t>(1, (total_cells + budget_cells - 1) / budget_cells);
+    const bool all_pairs = stride == 1;
+    std::vector<int32_t> pairs;
+    int64_t npairs = 0, cells = 0;
+    int64_t bin_cells[kBins] = {};
+    auto bin_of = [](int32_t q) {
+        int b = 0;
+        while (b + 1 < kBins && q >= kBinEdges[b + 1]) ++b;
+        return b;
+    };
+    for (size_t i = 0; i + 1 < (size_t)nseq; ++i) {
+        if (!all_pairs && (int64_t)i % stride != 0) continue;
+        const int64_t c = (int64_t)len(order[i]) * suffix[i + 1];
+        cells += c;
+        bin_cells[bin_of(len(order[i]))] += c;
+        npairs += (int64_t)nseq - 1 - (int64_t)i;
+        if (!all_pairs)
+            for (size_t j = i + 1; j < (size_t)nseq; ++j) {
+                pairs.push_back(std::max(order[i], order[j]));
+                pairs.push_back(std::min(order[i], order[j]));
+            }
+    }
+    if (cells == 0) return fail(MSA_ERR_INVALID_ARGUMENT, "the database has no cell to tune on");
+    if (npairs >= (int64_t(1) << 31)) return fail(MSA_ERR_TOO_LARGE, "tuning sample has too many pairs");
 
     msa::DeviceShape dev;
     MSA_TRY(msa::device_shape(dev));
@@ -333,135 +374,87 @@ int tune(const uint16_t* blocks, const int64_t* offset, const int32_t* size, int
     MSA_HIP(hipGetDevice(&device));
     hipDeviceProp_t prop;
     MSA_HIP(hipGetDeviceProperties(&prop, device));
-
-    std::vector<Measurement> meas;
-    int best[kBins];
-    double best_gcups[kBins], long_gcups[kBins];
-    for (int b = 0; b < kBins; ++b) {
-        best[b] = -1;
-        best_gcups[b] = long_gcups[b] = 0;
-    }
     int32_t* d_out = nullptr;
-    size_t out_cap = 0;
-    int rc = MSA_OK;
-    for (int b = 0; b < kBins && rc == MSA_OK; ++b) {
-        const int32_t lo = kBinEdges[b];
-        const int32_t hi = b + 1 < kBins ? kBinEdges[b + 1] : INT32_MAX;
-        std::vector<size_t> queries;
-        int64_t bin_cells = 0, bin_pairs = 0;
-        for (size_t i = 0; i < m; ++i) {
-            const int32_t len = seqs[order[i]].first_end;
-            if (len < lo || len >= hi || i + 1 == m) continue;
-            queries.push_back(i);
-            bin_cells += (int64_t)len * suffix_cells[i + 1];
-            bin_pairs += (int64_t)(m - 1 - i);
-        }
-        if (queries.empty()) continue;
-        // Every stride-th query (in length order) with all its targets.
-        const int64_t stride = std::max<int64_t>(1, std::min(bin_cells / budget_cells, bin_pairs / kMinSamplePairs));
-        std::vector<int32_t> pairs;
-        int64_t cells = 0;
-        for (size_t u = 0; u < queries.size(); u += (size_t)stride) {
-            const size_t i = queries[u];
-            const int32_t q = order[i];
-            for (size_t j = i + 1; j < m; ++j) {
-                const int32_t t = order[j];
-                pairs.push_back(std::max(q, t));
-                pairs.push_back(std::min(q, t));
-            }
-            cells += (int64_t)seqs[q].first_end * suffix_cells[i + 1];
-        }
-        const int64_t npairs = (int64_t)pairs.size() / 2;
-        if ((size_t)npairs > out_cap) {
-            if (d_out) (void)hipFree(d_out);
-            d_out = nullptr;
-            out_cap = (size_t)npairs;
-            rc = msa::hip_status(hipMalloc(&d_out, out_cap * sizeof(int32_t)), "hipMalloc");
-            if (rc != MSA_OK) break;
-        }
-        const int candidates = msa::kTileConfigCount + (lo >= 1024 ? 1 : 0);
-        for (int c = 0; c < candidates && rc == MSA_OK; ++c) {
-            msa_nw_options opt {};
-            const bool long_scheme = c == msa::kTileConfigCount;
-            if (long_scheme) {
-                opt.kernel = MSA_KERNEL_LONG;
-            } else {
-                opt.kernel = MSA_KERNEL_BATCHED;
-                opt.lanes_per_pair = msa::kTileConfig[c].lanes;
-                opt.rows_per_lane = msa::kTileConfig[c].rows;
-            }
-            double ms = 0;
-            rc = time_plan(blocks, offset, size, nseq, pairs, table625, penalty, opt, d_out, ms);
-            if (rc != MSA_OK) break;
-            const double gcups = ms > 0 ? (double)cells / (ms * 1e6) : 0;
-            meas.push_back({b, npairs, cells, long_scheme, long_scheme ? -1 : c, ms, gcups});
-            if (long_scheme) {
-                long_gcups[b] = gcups;
-            } else if (gcups > best_gcups[b]) {
-                best_gcups[b] = gcups;
-                best[b] = c;
-            }
-        }
-    }
-    if (d_out) (void)hipFree(d_out);
-    MSA_TRY(rc);
+    MSA_HIP(hipMalloc(&d_out, (size_t)npairs * sizeof(int32_t)));
 
-    // Bins without samples take the shape of the nearest measured bin below
-    // (or above, for the first ones).
-    int first_measured = -1;
-    for (int b = 0; b < kBins; ++b)
-        if (best[b] >= 0 && first_measured < 0) first_measured = b;
-    if (first_measured < 0) return fail(MSA_ERR_INVALID_ARGUMENT, "no clean pair to tune on");
-    int shape[kBins];
-    for (int b = 0, last = best[first_measured]; b < kBins; ++b) {
-        if (best[b] >= 0) last = best[b];
-        shape[b] = last;
+    std::vector<Trial> trials;
+    Trial cur;
+    for (int b = 0; b < kBins; ++b) cur.shape[b] = 0;
+    cur.long_threshold = INT32_MAX;
+    auto run = [&](Trial& t) {
+        return time_plan(blocks, offset, size, nseq, pairs, all_pairs, npairs, table625, penalty,
+                         make_tuning(t.shape, t.long_threshold), d_out, t.ms);
+    };
+    cur.step = "start";
+    int rc = run(cur);
+    if (rc == MSA_OK) trials.push_back(cur);
+    for (int b = 0; b < kBins && rc == MSA_OK; ++b) {
+        if (bin_cells[b] == 0) continue;
+        for (int c = 0; c < msa::kTileConfigCount && rc == MSA_OK; ++c) {
+            if (c == cur.shape[b]) continue;
+            Trial t = cur;
+            t.shape[b] = c;
+            char what[96];
+            std::snprintf(what, sizeof(what), "bin %d: %d lanes x %d rows", kBinEdges[b], msa::kTileConfig[c].lanes,
+                          msa::kTileConfig[c].rows);
+            t.step = what;
+            rc = run(t);
+            if (rc != MSA_OK) break;
+            trials.push_back(t);
+            if (t.ms < cur.ms * (1.0 - kMinGain)) cur = t;
+        }
     }
-    // The long scheme starts at the lowest bin edge from which every measured
-    // bin prefers it; with no such bin it starts past the longest measured bin.
-    int32_t long_threshold = msa::kDefaultLongThreshold;
-    int last_measured = -1;
-    for (int b = 0; b < kBins; ++b)
-        if (best[b] >= 0) last_measured = b;
-    int from = -1;
-    for (int b = last_measured; b >= 0; --b) {
-        if (best[b] < 0) continue;
-        if (long_gcups[b] > best_gcups[b]) from = b;
-        else break;
+    // Long thresholds: only those that move some pair.
+    const int32_t max_query = len(order[(size_t)nseq - 2]);   // the longest shorter side of a pair
+    std::vector<std::pair<int32_t, double>> curve;
+    curve.push_back({INT32_MAX, cur.ms});
+    for (int32_t th : kLongCandidates) {
+        if (rc != MSA_OK || th == INT32_MAX || th > max_query) continue;
+        Trial t = cur;
+        t.long_threshold = th;
+        t.step = "long from " + std::to_string(th);
+        rc = run(t);
+        if (rc != MSA_OK) break;
+        trials.push_back(t);
+        curve.push_back({th, t.ms});
     }
-    if (from >= 0) long_threshold = std::max<int32_t>(1, kBinEdges[from]);
-    else if (last_measured + 1 < kBins) long_threshold = std::max(long_threshold, kBinEdges[last_measured + 1]);
-    else long_threshold = INT32_MAX;
+    (void)hipFree(d_out);
+    MSA_TRY(rc);
+    Trial best = cur;
+    for (const auto& pt : curve)
+        if (pt.second < best.ms * (1.0 - kMinGain)) {
+            best.long_threshold = pt.first;
+            best.ms = pt.second;
+        }
 
     char name[260];
     std::snprintf(name, sizeof(name), "%s %s", prop.name, prop.gcnArchName);
     for (char* c = name; *c; ++c)
         if (*c == '"' || *c == '\\' || (unsigned char)*c < 0x20) *c = ' ';
+    const msa::Tuning table = make_tuning(best.shape, best.long_threshold);
     json.clear();
     appendf(json, "{\n  \"format\": \"msa-nw-tuning/1\",\n  \"device\": \"%s\",\n  \"compute_units\": %d,\n", name,
             dev.compute_units);
-    appendf(json, "  \"budget_cells\": %lld,\n  \"long_threshold\": %d,\n  \"bins\": [", (long long)budget_cells,
-            long_threshold);
-    bool first = true;
-    for (int b = 0; b < kBins; ++b) {
-        if (b > 0 && shape[b] == shape[b - 1]) continue;
-        appendf(json, "%s\n    {\"min_query\": %d, \"lanes\": %d, \"rows\": %d}", first ? "" : ",", kBinEdges[b],
-                msa::kTileConfig[shape[b]].lanes, msa::kTileConfig[shape[b]].rows);
-        first = false;
-    }
+    appendf(json, "  \"method\": \"coordinate descent over executions of the sample\",\n");
+    appendf(json, "  \"sample\": {\"pairs\": %lld, \"cells\": %lld, \"all_pairs\": %s},\n", (long long)npairs,
+            (long long)cells, all_pairs ? "true" : "false");
+    appendf(json, "  \"long_threshold\": %d,\n  \"bins\": [", table.long_threshold);
+    for (size_t b = 0; b < table.bins.size(); ++b)
+        appendf(json, "%s\n    {\"min_query\": %d, \"lanes\": %d, \"rows\": %d}", b ? "," : "", table.bins[b].min_query,
+                msa::kTileConfig[table.bins[b].config].lanes, msa::kTileConfig[table.bins[b].config].rows);
+    appendf(json, "\n  ],\n  \"bin_cells\": [");
+    for (int b = 0; b < kBins; ++b)
+        appendf(json, "%s\n    {\"min_query\": %d, \"cells\": %lld, \"lanes\": %d, \"rows\": %d}", b ? "," : "",
+                kBinEdges[b], (long long)bin_cells[b], msa::kTileConfig[best.shape[b]].lanes,
+                msa::kTileConfig[best.shape[b]].rows);
+    appendf(json, "\n  ],\n  \"threshold_trials\": [");
+    for (size_t i = 0; i < curve.size(); ++i)
+        appendf(json, "%s\n    {\"long_threshold\": %d, \"ms\": %.4f}", i ? "," : "", curve[i].first, curve[i].second);
     appendf(json, "\n  ],\n  \"measurements\": [");
-    for (size_t i = 0; i < meas.size(); ++i) {
-        const Measurement& x = meas[i];
-        const int32_t hi = x.bin + 1 < kBins ? kBinEdges[x.bin + 1] : -1;
-        appendf(json, "%s\n    {\"min_query\": %d, \"max_query\": %d, \"pairs\": %lld, \"cells\": %lld, ",
-                i ? "," : "", kBinEdges[x.bin], hi < 0 ? -1 : hi - 1, (long long)x.pairs, (long long)x.cells);
-        if (x.long_scheme)
-            appendf(json, "\"scheme\": \"long\", \"lanes\": 64, \"rows\": 0, ");
-        else
-            appendf(json, "\"scheme\": \"batched\", \"lanes\": %d, \"rows\": %d, ", msa::kTileConfig[x.config].lanes,
-                    msa::kTileConfig[x.config].rows);
-        appendf(json, "\"ms\": %.4f, \"gcups\": %.1f}", x.ms, x.gcups);
-    }
+    for (size_t i = 0; i < trials.size(); ++i)
+        appendf(json, "%s\n    {\"step\": \"%s\", \"long_threshold\": %d, \"ms\": %.4f, \"gcups\": %.1f}",
+                i ? "," : "", trials[i].step.c_str(), trials[i].long_threshold, trials[i].ms,
+                trials[i].ms > 0 ? (double)cells / (trials[i].ms * 1e6) : 0.0);
     appendf(json, "\n  ]\n}\n");
     return MSA_OK;
 }

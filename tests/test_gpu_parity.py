@@ -6,6 +6,7 @@ full-size workloads are checked through properties that do not depend on a
 CPU pass over all pairs: batched == plain on every pair, a random oracle
 sample, and symmetry.  The bar is bit-exact int32 == the reference's float.
 """
+import json
 import zlib
 
 import numpy as np
@@ -297,31 +298,38 @@ def test_unknown_tile_shape_rejected():
 
 
 def test_tuner_writes_a_table_that_plans_read(tmp_path):
-    """msa_nw_tune on a small uneven database: one measurement per tile shape
-    per populated length bin (plus the long scheme from 1024 up), a table
-    that msa_nw_options.tuning_file reads, and scores that still match the
-    oracle under it."""
+    """msa_nw_tune on a small uneven database: coordinate descent over whole
+    executions (every other tile shape for each populated length bin, then
+    every long threshold up to the longest query), a table that
+    msa_nw_options.tuning_file reads, and scores that still match the oracle
+    under it."""
     rng = np.random.default_rng(12)
     T, p = H.tables()["blosum62"]
     lengths = np.clip(np.exp(rng.normal(np.log(300), 0.9, 48)), 20, 2500).astype(int)
     codes = [rng.integers(0, 20, int(n)).astype(np.uint8) for n in lengths]
     db = W.pack(codes)
     path = str(tmp_path / "tuning.json")
-    table = N.tune(*db, T, p, budget_cells=int(2e8), path=path)
+    table = N.tune(*db, T, p, path=path)
     assert table["format"] == "msa-nw-tuning/1"
+    assert table["sample"]["all_pairs"] and table["sample"]["pairs"] == 48 * 47 // 2
     assert table["bins"][0]["min_query"] == 0
     mins = [b["min_query"] for b in table["bins"]]
     assert mins == sorted(set(mins))
     for b in table["bins"]:
         assert (b["lanes"], b["rows"]) in TILES
-    shapes = {}
-    for m in table["measurements"]:
-        shapes.setdefault(m["min_query"], set()).add((m["scheme"], m["lanes"], m["rows"]))
-        assert m["gcups"] > 0 and m["ms"] > 0 and m["pairs"] > 0
-    for lo, got in shapes.items():
-        want = {("batched", a, b) for a, b in TILES} | ({("long", 64, 0)} if lo >= 1024 else set())
-        assert got == want, (lo, got)
-    assert table["long_threshold"] >= 1
+    populated = [b for b in table["bin_cells"] if b["cells"] > 0]
+    assert sum(b["cells"] for b in table["bin_cells"]) == table["sample"]["cells"]
+    steps = [m["step"] for m in table["measurements"]]
+    assert steps[0] == "start"
+    assert len([s for s in steps if s.startswith("bin ")]) == 2 * len(populated)
+    tried = [t["long_threshold"] for t in table["threshold_trials"]]
+    assert tried == [2 ** 31 - 1] + [th for th in (1024, 2048, 4096, 8192, 16384) if th <= np.sort(lengths)[-2]]
+    # A threshold is adopted only when it saves more than 0.5% of the time.
+    times = {t["long_threshold"]: t["ms"] for t in table["threshold_trials"]}
+    assert table["long_threshold"] in times
+    if table["long_threshold"] != 2 ** 31 - 1:
+        assert times[table["long_threshold"]] < 0.995 * times[2 ** 31 - 1]
+    assert all(m["ms"] > 0 for m in table["measurements"])
     with open(path) as f:
-        assert f.read().startswith("{")
+        assert json.load(f)["bins"] == table["bins"]
     assert_oracle(db, T, p, kernel="auto", tuning_file=path)
