@@ -217,7 +217,15 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
             if (lx != ly) return lx > ly;
             return x.index < y.index;
         });
-        constexpr int kItemTargets = 16;
+        // Targets per item: 16 (four rounds of four) when the plan has enough
+        // pairs to give every resident wave four items, else 8 or 4, so that
+        // small plans (serving jobs) still spread over the whole GPU.
+        int per_cu = 0;
+        MSA_TRY(batched_occupancy(&per_cu));
+        if (per_cu < 1) return fail(MSA_ERR_RUNTIME, "batched kernel does not fit a compute unit");
+        const int64_t waves = (int64_t)per_cu * dev.compute_units * (batched_threads() / 64);
+        int item_targets = 16;
+        while (item_targets > 4 && (int64_t)fast.size() < (int64_t)item_targets * 4 * waves) item_targets /= 2;
         struct Item { int4 v; int64_t cost; };
         std::vector<Item> items;
         hp.targets.resize(fast.size());
@@ -242,8 +250,8 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
             const int32_t flags = (fast[k].query & 1) | (cfg << 1);
             // Rows computed: the query rounded up to 4 rows per lane.
             const int64_t rows = ((int64_t)lq + 4 * tc.lanes - 1) / (4 * tc.lanes) * (4 * tc.lanes);
-            for (size_t s = k; s < e; s += kItemTargets) {
-                const int32_t cnt = (int32_t)std::min<size_t>(kItemTargets, e - s);
+            for (size_t s = k; s < e; s += item_targets) {
+                const int32_t cnt = (int32_t)std::min<size_t>(item_targets, e - s);
                 int64_t cost = 0;
                 // A round of `groups` targets costs as much as its longest (the first).
                 for (size_t u = s; u < s + cnt; u += groups) {
@@ -257,9 +265,6 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
         std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.cost > y.cost; });
         hp.items.resize(items.size());
         for (size_t u = 0; u < items.size(); ++u) hp.items[u] = items[u].v;
-        int per_cu = 0;
-        MSA_TRY(batched_occupancy(&per_cu));
-        if (per_cu < 1) return fail(MSA_ERR_RUNTIME, "batched kernel does not fit a compute unit");
         hp.batch_grid = (int)std::min<int64_t>((int64_t)per_cu * dev.compute_units, ((int64_t)items.size() + 3) / 4);
         // Lines are touched only by queries of more than one strip.
         hp.batch_stride = need_lines ? ((max_t + 63) & ~63) : 0;
