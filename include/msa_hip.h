@@ -214,6 +214,37 @@ int msa_nw_plain(const uint16_t* d_blocks, const int64_t* d_offset, const int32_
                  const int32_t* table625, int32_t penalty,
                  int32_t* d_out, msa_stream stream);
 
+/* ---- phylogeny: neighbor joining (phylogeny::run, src/phylogeny/phylogeny.cuh:
+ *      110-124; njoining::sequential_{symmetric,linear},
+ *      src/phylogeny/njoining/impl/sequential.cpp:270-368) ---------------------
+ * The stage after the pairwise path: it consumes the distance matrix as the
+ * pairwise stage leaves it (scores in pair order, larger = closer) and builds
+ * the guide tree by neighbor joining with the reference's sequential
+ * semantics, float for float.  Nodes 0..n-1 are the leaves (OTU k = sequence
+ * k), node n+t is the t-th join, the root is node 2n-2 (src/phylogeny/
+ * tree.cuh, njoining/star.cuh:63-80). */
+enum {
+    MSA_NJ_SYMMETRIC = 0,    /* njoining::*_symmetric: the reference's default */
+    MSA_NJ_LINEAR = 1        /* njoining::*_linear */
+};
+
+typedef struct {
+    uint32_t parent;         /* 0xFFFFFFFF for the root */
+    uint32_t child[2];       /* smaller node id first; 0xFFFFFFFF for leaves */
+    float distance;          /* branch length to the parent; FLT_MAX for the root */
+    uint32_t level;          /* 0 for leaves, else 1 + the larger child level */
+} msa_nj_node;
+
+/* Host in, host out: distances holds n(n-1)/2 floats in pair order
+ * (pairwise::distance_matrix's buffer), nodes receives 2n-1 nodes (nothing
+ * for n < 2).  MSA_ERR_TOO_LARGE above 65535 OTUs. */
+int msa_nj_build(const float* distances, int32_t n, int32_t variant, msa_nj_node* nodes);
+/* Device in, device out, chained after a pairwise plan: d_scores holds the
+ * plan's n(n-1)/2 int32 scores in pair order, d_nodes 2n-1 nodes.  Returns
+ * after the tree is built (the working buffers are released). */
+int msa_nj_build_device(const int32_t* d_scores, int32_t n, int32_t variant, msa_nj_node* d_nodes,
+                        msa_stream stream);
+
 /* ---- serving -----------------------------------------------------------------
  * No counterpart in the reference (a batch MPI pipeline); its closest relative
  * is the free-memory admission of hybrid.cu:345-375.  A server keeps a device

@@ -30,6 +30,13 @@ KERNEL_PLAIN = 1
 KERNEL_BATCHED = 2
 KERNEL_LONG = 3
 
+NJ_SYMMETRIC = 0
+NJ_LINEAR = 1
+NJ_UNDEFINED = 0xFFFFFFFF
+# msa_nj_node
+NJ_NODE = np.dtype([("parent", np.uint32), ("child", np.uint32, (2,)), ("distance", np.float32),
+                    ("level", np.uint32)])
+
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64
@@ -108,6 +115,8 @@ SIGNATURES = {
     "msa_nw_align": (ctypes.c_int, [_vp, _vp, _vp, _i32, _vp, _i64, _vp, _i32, ctypes.POINTER(NwOptions), _vp]),
     "msa_nw_plain": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _vp, _i64, _vp, _i32, _vp, _vp]),
     "msa_nw_tune": (ctypes.c_int, [_vp, _vp, _vp, _i32, _vp, _i32, _i64, ctypes.c_char_p, _vp, ctypes.c_size_t]),
+    "msa_nj_build": (ctypes.c_int, [_vp, _i32, _i32, _vp]),
+    "msa_nj_build_device": (ctypes.c_int, [_vp, _i32, _i32, _vp, _vp]),
     "msa_srv_open": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(SrvConfig)]),
     "msa_srv_submit": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, ctypes.POINTER(ctypes.c_uint64)]),
     "msa_srv_poll": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]),
@@ -292,6 +301,22 @@ def tune(blocks, offsets, sizes, table625, penalty, budget_cells=0, path=None):
                             int(penalty), int(budget_cells), os.fsencode(path) if path else None,
                             report, len(report)))
     return json.loads(report.value.decode())
+
+
+def nj_build(distances, n, variant=NJ_SYMMETRIC):
+    """msa_nj_build: neighbor joining of n OTUs from n(n-1)/2 float distances
+    in pair order; returns the 2n-1 nodes (NJ_NODE records)."""
+    d = np.ascontiguousarray(distances, dtype=np.float32)
+    if d.size != n * (n - 1) // 2:
+        raise ValueError("expected %d distances for %d OTUs, got %d" % (n * (n - 1) // 2, n, d.size))
+    nodes = np.zeros(max(2 * n - 1, 0) if n >= 2 else 0, dtype=NJ_NODE)
+    check(lib().msa_nj_build(_ptr(d), int(n), int(variant), _ptr(nodes)))
+    return nodes
+
+
+def nj_build_device(d_scores, n, variant, d_nodes, stream=None):
+    """msa_nj_build_device on caller-owned device buffers (int32 scores in, nodes out)."""
+    check(lib().msa_nj_build_device(d_scores, int(n), int(variant), d_nodes, stream))
 
 
 class Server:

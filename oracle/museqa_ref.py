@@ -16,6 +16,14 @@ Contents and what each follows:
                                           with a linear gap (full matrix, int64)
   CpuOracle ............................. ctypes wrapper of nw_oracle.c, the C
                                           restatement of sequential.cpp:29-95
+  NjOracle .............................. ctypes wrapper of nj_oracle.c, the C
+                                          restatement of the sequential
+                                          neighbor joining
+                                          (src/phylogeny/njoining/impl/
+                                          sequential.cpp)
+  tree_splits / additive_tree ........... textbook helpers that pin NjOracle:
+                                          the bipartitions of a tree and the
+                                          path-length metric of a random tree
 """
 import ctypes
 import json
@@ -250,3 +258,113 @@ class ReferenceBuild:
                                     nblocks.ctypes.data, pairs.ctypes.data, len(pairs), table.ctypes.data,
                                     float(penalty), out.ctypes.data, int(workers))
         return out
+
+
+# msa_nj_node / nj_node (oracle/nj_oracle.c)
+NJ_NODE = np.dtype([("parent", np.uint32), ("child", np.uint32, (2,)), ("distance", np.float32),
+                    ("level", np.uint32)])
+NJ_UNDEFINED = 0xFFFFFFFF
+
+
+class NjOracle:
+    """ctypes wrapper of nj_oracle.c (in oracle/_build/liboracle.so)."""
+
+    def __init__(self, path=None):
+        path = path or os.path.join(HERE, "_build", "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle library not built: run `make -C oracle`")
+        self.lib = ctypes.CDLL(path)
+        self.lib.nj_oracle_planted.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                               ctypes.c_void_p]
+        self.lib.nj_oracle_planted.restype = ctypes.c_int
+
+    def build(self, distances, n, variant=0, mistake=0):
+        """Nodes (NJ_NODE records, 2n-1) of the tree over n(n-1)/2 float
+        distances in pair order; variant 0 = symmetric, 1 = linear.  A non-zero
+        `mistake` plants an error (tests of the pins only)."""
+        d = np.ascontiguousarray(distances, dtype=np.float32)
+        assert d.size == n * (n - 1) // 2
+        nodes = np.zeros(2 * n - 1 if n >= 2 else 0, dtype=NJ_NODE)
+        if n >= 2 and self.lib.nj_oracle_planted(d.ctypes.data, n, variant, mistake, nodes.ctypes.data) != 0:
+            raise RuntimeError("oracle failed")
+        return nodes
+
+
+def tree_splits(children, n):
+    """Bipartitions of the leaves 0..n-1 of a tree, each as the frozenset of
+    the side that does not hold leaf 0; `children` maps an internal node to
+    its children.  Trivial splits (one leaf, or all but one) are left out."""
+    below = {}
+
+    def leaves(v):
+        if v < n:
+            return frozenset([v])
+        if v not in below:
+            below[v] = frozenset().union(*[leaves(c) for c in children[v]])
+        return below[v]
+
+    full = frozenset(range(n))
+    out = set()
+    for v in children:
+        side = leaves(v)
+        if 0 in side:
+            side = full - side
+        if 1 < len(side) < n - 1:
+            out.add(side)
+    return out
+
+
+def nj_splits(nodes, n):
+    """tree_splits of an NJ_NODE tree."""
+    children = {v: [int(c) for c in nodes[v]["child"]] for v in range(n, len(nodes))}
+    return tree_splits(children, n)
+
+
+def additive_tree(rng, n, lo=1, hi=20):
+    """A random unrooted binary tree on leaves 0..n-1 with integer edge
+    lengths in [lo, hi]; returns (packed path-length distances in pair order
+    as float32, its splits, {frozenset side: edge length} for every edge)."""
+    # Random topology by sequential insertion: each new leaf splits an edge.
+    edges = {(0, n), (1, n), (2, n)}        # node n is the first internal node
+    nxt = n + 1
+    for leaf in range(3, n):
+        a, b = sorted(edges)[int(rng.integers(0, len(edges)))]
+        edges.remove((a, b))
+        mid = nxt
+        nxt += 1
+        edges |= {(a, mid), (b, mid), (leaf, mid)}
+    length = {e: int(rng.integers(lo, hi + 1)) for e in edges}
+    adj = {}
+    for (a, b), w in length.items():
+        adj.setdefault(a, []).append((b, w))
+        adj.setdefault(b, []).append((a, w))
+    dist = np.zeros((n, n), dtype=np.int64)
+    side_len = {}
+    for src in range(n):
+        stack, seen = [(src, 0)], {src}
+        while stack:
+            v, d = stack.pop()
+            if v < n:
+                dist[src, v] = d
+            for u, w in adj[v]:
+                if u not in seen:
+                    seen.add(u)
+                    stack.append((u, d + w))
+    # the split of each edge: leaves on the far side of (a, b) from a
+    for (a, b), w in length.items():
+        stack, seen, side = [b], {a, b}, set()
+        while stack:
+            v = stack.pop()
+            if v < n:
+                side.add(v)
+            for u, _ in adj[v]:
+                if u not in seen:
+                    seen.add(u)
+                    stack.append(u)
+        side = frozenset(side)
+        if 0 in side:
+            side = frozenset(range(n)) - side
+        side_len[side] = w
+    packed = np.array([dist[i, j] for i in range(1, n) for j in range(i)], dtype=np.float32)
+    splits = {s for s in side_len if 1 < len(s) < n - 1}
+    return packed, splits, side_len

@@ -136,3 +136,15 @@ def test_valid_tuning_table_is_accepted(tmp_path):
                                        '"bins": [{"min_query": 0, "lanes": 16, "rows": 32}, '
                                        '{"min_query": 256, "lanes": 32, "rows": 16}], "measurements": []}')
     assert rc != N.MSA_ERR_INVALID_ARGUMENT, msg
+
+
+def test_nj_argument_errors_before_any_device_call():
+    lib = N.lib()
+    nodes = np.zeros(1, dtype=N.NJ_NODE)
+    assert lib.msa_nj_build(None, 70000, 0, None) == N.MSA_ERR_TOO_LARGE
+    assert lib.msa_nj_build(None, 5, 7, None) == N.MSA_ERR_INVALID_ARGUMENT
+    assert lib.msa_nj_build(None, -1, 0, None) == N.MSA_ERR_INVALID_ARGUMENT
+    assert lib.msa_nj_build_device(None, 5, 2, None, None) == N.MSA_ERR_INVALID_ARGUMENT
+    # Fewer than two OTUs: no tree, nothing to do.
+    assert lib.msa_nj_build(None, 1, 0, nodes.ctypes.data) == N.MSA_OK
+    assert lib.msa_nj_build(None, 0, 1, None) == N.MSA_OK
