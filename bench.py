@@ -250,6 +250,55 @@ def dominant_kernel(info):
     return max(counts, key=lambda k: counts[k])
 
 
+def phylogeny_run(N, w, steps, warmup, cpu=True, variant=0):
+    """Neighbor joining over the workload's pairwise scores, both on the GPU:
+    the scores stay in device memory and msa_nj_build_device builds the tree.
+    The oracle builds the same tree on one host core (the CPU baseline, kind
+    "port") and the two trees must be identical."""
+    from oracle import museqa_ref as M
+    L = N.lib()
+    n = w.nseq
+    plan = N.Plan(w.blocks, w.offsets, w.sizes, w.table, w.penalty)
+    d_scores = N.DeviceBuffer(4 * w.npairs)
+    plan.execute(d_scores.ptr)
+    plan.status()
+    d_nodes = N.DeviceBuffer(N.NJ_NODE.itemsize * (2 * n - 1))
+    for _ in range(warmup):
+        N.nj_build_device(d_scores.ptr, n, variant, d_nodes.ptr)
+    N.check(L.msa_device_sync())
+    t0 = time.time()
+    for _ in range(steps):
+        N.nj_build_device(d_scores.ptr, n, variant, d_nodes.ptr)
+    ms = 1e3 * (time.time() - t0) / steps
+    nodes = d_nodes.download(N.NJ_NODE, 2 * n - 1)
+    q_evals = sum(c * (c - 1) // 2 for c in range(2, n + 1))
+    result = {
+        "metric": "neighbor-joining tree build on 1 MI355X (after the pairwise stage)", "value": round(ms, 3),
+        "unit": "ms per tree", "n_gpus": 1, "steps": steps, "warmup": warmup, "higher_is_better": False,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": w.name, "otus": n, "variant": "symmetric" if variant == 0 else "linear",
+                   "joins": n - 1, "q_evaluations": q_evals},
+        "q_evaluations_per_s": round(q_evals / (ms * 1e-3), 1),
+        "roofline": {"bound": "hbm", "achieved": round(4.0 * q_evals / (ms * 1e-3) / 1e9, 1), "peak": 8000.0,
+                     "unit": "GB/s", "frac": round(4.0 * q_evals / (ms * 1e-3) / 8e12, 4), "traffic": None,
+                     "bytes_per_q": 4},
+    }
+    if cpu:
+        scores = d_scores.download(np.int32, w.npairs).astype(np.float32)
+        t0 = time.time()
+        exp = M.NjOracle().build(scores, n, variant)
+        cpu_s = time.time() - t0
+        same = all(np.array_equal(nodes[f], exp[f]) for f in ("parent", "child", "level")) and bool(np.all(
+            (nodes["distance"].view(np.uint32) == exp["distance"].view(np.uint32)) |
+            (np.isnan(nodes["distance"]) & np.isnan(exp["distance"]))))
+        result["bit_exact_vs_oracle"] = same
+        result["cpu_baseline"] = {"value": round(1e3 * cpu_s, 1), "unit": "ms per tree", "cores": 1, "kind": "port",
+                                  "sample": "the whole tree over the same scores (oracle/nj_oracle.c, -O3)"}
+        result["vs_cpu"] = round(1e3 * cpu_s / ms, 1)
+    plan.close()
+    return result
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -264,6 +313,8 @@ def main():
     ap.add_argument("--rates", default="10,20,50,100,200,500")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-rate", type=float, default=100.0, help="rate of the serving figure in the main line")
+    ap.add_argument("--phylogeny", action="store_true",
+                    help="only the neighbor-joining stage over the workload's scores (its own JSON line)")
     ap.add_argument("--tuning", default=DEFAULT_TUNING,
                     help="tuning table for the batched plan (default: the committed gfx950 table; '' = built-in)")
     args = ap.parse_args()
@@ -288,6 +339,10 @@ def main():
             "serving": sweep}), flush=True)
         return 0 if all(r["bit_exact_vs_alone"] for r in sweep) else 1
     w = make_workload(args.workload, tables)
+    if args.phylogeny:
+        res = phylogeny_run(N, w, args.steps, args.warmup, cpu=not args.no_cpu)
+        print(json.dumps(res), flush=True)
+        return 0 if res.get("bit_exact_vs_oracle", True) else 1
     cells = w.cells()
     log("workload:", w.description, "pairs", w.npairs, "cells %.4e" % cells, "checksum", w.checksum())
 

@@ -19,15 +19,18 @@
 //     variant, the reference's default) or a swap with the last index and its
 //     removal (linear variant) (:188-221).
 //
-// Layout: the matrix is n x n float in HBM and never moves.  The reference's
-// swaps and removals are relabelings, so logical index k of an iteration maps
-// to a physical row through `slot`; the symmetric variant's "remove index 0"
-// is a shift of the base of the logical arrays (slot, node ids, row sums) by
-// one per iteration.  Per iteration: one grid-wide arg-max of Q over the
-// count(count-1)/2 logical pairs (64-bit keys: order-preserving Q bits, then
-// the complement of the pair index, so ties keep the first pair), and one
-// workgroup that joins the pair.  The row sums of the joined OTU are a
-// sequential float sum (the reference's order), done by one lane from LDS.
+// Layout: the matrix is n x n float in HBM, kept in logical order so that the
+// arg-max streams rows.  The reference's "swap y with index 0, remove index 0"
+// (symmetric variant) is a copy of row and column 0 into y and a shift of the
+// base of the logical window (matrix, node ids, row sums) by one row and one
+// column; "swap y with the last, remove the last" (linear) is a copy of the
+// last row and column into y.  Three launches per join, no atomics between
+// workgroups: every workgroup of the arg-max leaves the best key of its rows
+// (64-bit keys: order-preserving Q bits, then the complement of the pair
+// index, so ties keep the first pair); the first join kernel reduces the keys
+// and computes the joined OTU's distances and the row sums over all OTUs; the
+// second writes rows and columns while one lane sums the joined OTU's row in
+// the reference's order from LDS.
 #include "msa_internal.hpp"
 
 #include <algorithm>
@@ -38,18 +41,19 @@ namespace msa {
 
 namespace {
 
-constexpr int kJoinThreads = 1024;
-constexpr int kSumChunk = 4096;          // doubles staged in LDS for the sequential sum
-constexpr int kArgmaxThreads = 256;
+constexpr int kStepThreads = 256;
+constexpr int kStage = 8192;             // floats staged in LDS for the sequential sum
+constexpr int kMaxStepBlocks = 2048;     // partial arg-max slots
 constexpr uint32_t kUndefined = 0xFFFFFFFFu;
 
 struct NjState {
-    float* m;                     // n x n, physical rows
-    int32_t* slot;                // logical -> physical row (from base)
+    float* m;                     // n x n; logical (i, j) at (base + i, base + j)
     uint32_t* map;                // logical -> node id (from base)
     float* cache;                 // logical row sums (from base)
     double* upd;                  // the joined OTU's distances, this iteration
-    unsigned long long* best;     // arg-max key of this iteration
+    unsigned long long* partial;  // per workgroup: its arg-max key of this iteration
+    int32_t* pick;                // the pair joined this iteration (x, y)
+    int32_t* inexact;             // some distance of this join is not a float
     int32_t* error;               // set when no pair has a finite Q
     msa_nj_node* nodes;           // 2n-1
     int32_t n;
@@ -69,6 +73,22 @@ __device__ __forceinline__ uint32_t float_key(float v)
 {
     const uint32_t u = __float_as_uint(v);
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// Maximum over the workgroup (every thread gets it).
+__device__ unsigned long long block_max(unsigned long long v)
+{
+    __shared__ unsigned long long part[kStepThreads / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off);
+        v = o > v ? o : v;
+    }
+    __syncthreads();
+    if (threadIdx.x % 64 == 0) part[threadIdx.x / 64] = v;
+    __syncthreads();
+    v = part[0];
+    for (int w = 1; w < kStepThreads / 64; ++w) v = part[w] > v ? part[w] : v;
+    return v;
 }
 
 template <typename T>
@@ -91,7 +111,6 @@ __global__ void init_kernel(NjState s)
     const int32_t n = s.n;
     for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < 2 * n - 1; k += gridDim.x * blockDim.x) {
         if (k < n) {
-            s.slot[k] = k;
             s.map[k] = (uint32_t)k;
             // Row sum in index order: sum_{j != k} d(j, k), read down column k
             // (the matrix is symmetric), coalesced across threads.
@@ -108,26 +127,27 @@ __global__ void init_kernel(NjState s)
         s.nodes[k] = node;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        *s.best = 0;
         *s.error = 0;
+        *s.inexact = 0;
     }
 }
 
-// Arg-max of Q over the count(count-1)/2 logical pairs of this iteration:
-// workgroups take rows i = 1.. round-robin, lanes run along j < i.
-__global__ __launch_bounds__(kArgmaxThreads) void argmax_kernel(NjState s, int32_t count, int32_t base)
+// Arg-max of Q over the count(count-1)/2 logical pairs: workgroups take rows
+// i = 1.. round-robin, lanes run along j < i; each workgroup leaves its key.
+__global__ __launch_bounds__(kStepThreads) void argmax_kernel(NjState s, int32_t count, int32_t base)
 {
-    const int32_t* slot = s.slot + base;
+    const int64_t n = s.n;
+    const float* mb = s.m + (int64_t)base * n + base;
     const float* cache = s.cache + base;
     const float factor = (float)(count - 2);
-    const int64_t n = s.n;
     unsigned long long best = 0;
     for (int32_t i = 1 + blockIdx.x; i < count; i += gridDim.x) {
-        const float* row = s.m + (int64_t)slot[i] * n;
+        const float* row = mb + (int64_t)i * n;
         const float ci = cache[i];
         const uint32_t c0 = (uint32_t)((int64_t)i * (i - 1) / 2);
-        for (int32_t j = threadIdx.x; j < i; j += kArgmaxThreads) {
-            float q = factor * row[slot[j]] - ci - cache[j];
+#pragma unroll 4
+        for (int32_t j = threadIdx.x; j < i; j += kStepThreads) {
+            float q = factor * row[j] - ci - cache[j];
             if (q > -INFINITY) {                     // NaN and -inf never win (the reference starts at -inf)
                 q = q + 0.0f;                        // -0 and +0 compare equal: one key for both
                 const unsigned long long key =
@@ -136,105 +156,156 @@ __global__ __launch_bounds__(kArgmaxThreads) void argmax_kernel(NjState s, int32
             }
         }
     }
-    // Wave, then workgroup, then device maximum.
-    for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long o = __shfl_xor(best, off);
-        best = o > best ? o : best;
-    }
-    __shared__ unsigned long long part[kArgmaxThreads / 64];
-    if (threadIdx.x % 64 == 0) part[threadIdx.x / 64] = best;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < kArgmaxThreads / 64; ++w) best = part[w] > best ? part[w] : best;
-        if (best) atomicMax(s.best, best);
-    }
+    best = block_max(best);
+    if (threadIdx.x == 0) s.partial[blockIdx.x] = best;
 }
 
-// Joins the chosen pair (one workgroup).
-__global__ __launch_bounds__(kJoinThreads) void join_kernel(NjState s, int32_t count, int32_t base, uint32_t parent,
-                                                            int32_t variant)
+// Reduces the arg-max partials, decodes the pair, and (every workgroup, on
+// its share of the OTUs) computes the joined OTU's distances and updates the
+// row sums; workgroup 0 also records the join in the tree.
+__global__ __launch_bounds__(kStepThreads) void join_a_kernel(NjState s, int32_t count, int32_t base, uint32_t parent,
+                                                              int32_t partials)
 {
-    __shared__ int32_t sx, sy, px, py;
+    __shared__ int32_t sx, sy;
     __shared__ float sdxy;
-    __shared__ double stage[kSumChunk];
-    int32_t* slot = s.slot + base;
+    const int64_t n = s.n;
+    float* mb = s.m + (int64_t)base * n + base;
     uint32_t* map = s.map + base;
     float* cache = s.cache + base;
-    const int64_t n = s.n;
+    unsigned long long key = 0;
+    for (int32_t b = threadIdx.x; b < partials; b += kStepThreads) key = s.partial[b] > key ? s.partial[b] : key;
+    key = block_max(key);
     if (threadIdx.x == 0) {
-        const unsigned long long key = *s.best;
         // No finite Q (the reference's pick is undefined there): flag it and
         // join the first pair so that every access stays in bounds.
-        if (key == 0) *s.error = 1;
         const int64_t c = key ? (int64_t)(0xFFFFFFFFu - (uint32_t)(key & 0xFFFFFFFFu)) : 0;
         const int64_t i = pair_row(c);
         const int32_t x = (int32_t)i, y = (int32_t)(c - i * (i - 1) / 2);
-        const float dxy = s.m[(int64_t)slot[x] * n + slot[y]];
-        const float pairsum = cache[x] - cache[y];
-        const float dx = (float)(.5 * (double)dxy + (double)(pairsum / (float)(2 * (count - 2))));
-        const float dy = dxy - dx;
-        // star::join (src/phylogeny/njoining/star.cuh:63-80)
-        const uint32_t a = map[x], b = map[y];
-        const uint32_t c1 = a < b ? a : b, c2 = a < b ? b : a;
-        s.nodes[c1].distance = a < b ? dx : dy;
-        s.nodes[c1].parent = parent;
-        s.nodes[c2].distance = a < b ? dy : dx;
-        s.nodes[c2].parent = parent;
-        const uint32_t l1 = s.nodes[c1].level, l2 = s.nodes[c2].level;
-        s.nodes[parent].level = (l1 > l2 ? l1 : l2) + 1;
-        s.nodes[parent].child[0] = c1;
-        s.nodes[parent].child[1] = c2;
-        sx = x, sy = y, px = slot[x], py = slot[y], sdxy = dxy;
+        const float dxy = mb[(int64_t)x * n + y];
+        sx = x, sy = y, sdxy = dxy;
+        if (blockIdx.x == 0) {
+            if (key == 0) *s.error = 1;
+            const float pairsum = cache[x] - cache[y];
+            const float dx = (float)(.5 * (double)dxy + (double)(pairsum / (float)(2 * (count - 2))));
+            const float dy = dxy - dx;
+            // star::join (src/phylogeny/njoining/star.cuh:63-80)
+            const uint32_t a = map[x], b = map[y];
+            const uint32_t c1 = a < b ? a : b, c2 = a < b ? b : a;
+            s.nodes[c1].distance = a < b ? dx : dy;
+            s.nodes[c1].parent = parent;
+            s.nodes[c2].distance = a < b ? dy : dx;
+            s.nodes[c2].parent = parent;
+            const uint32_t l1 = s.nodes[c1].level, l2 = s.nodes[c2].level;
+            s.nodes[parent].level = (l1 > l2 ? l1 : l2) + 1;
+            s.nodes[parent].child[0] = c1;
+            s.nodes[parent].child[1] = c2;
+            s.pick[0] = x;
+            s.pick[1] = y;
+        }
     }
     __syncthreads();
     const int32_t x = sx, y = sy;
-    const int64_t rx = px, ry = py;
     const float dxy = sdxy;
-    for (int32_t i = threadIdx.x; i < count; i += kJoinThreads) {
-        const int64_t ri = slot[i];
-        const float previous = s.m[ri * n + rx] + s.m[ri * n + ry];
+    const float* rowx = mb + (int64_t)x * n;
+    const float* rowy = mb + (int64_t)y * n;
+    // The row sums of x and y are read above for the branch lengths; their
+    // updates here are dead in the reference (x's is replaced by the new sum,
+    // y leaves or is replaced by `other`'s), so they are skipped, which also
+    // keeps workgroup 0's reads free of races.
+    int32_t inexact = 0;
+    for (int32_t i = blockIdx.x * kStepThreads + threadIdx.x; i < count; i += gridDim.x * kStepThreads) {
+        const float previous = rowx[i] + rowy[i];
         const double updated = (double)(previous - dxy) * .5;
         s.upd[i] = updated;
-        cache[i] = (float)((double)cache[i] + (updated - (double)previous));
+        inexact |= (double)(float)updated != updated;
+        if (i != x && i != y) cache[i] = (float)((double)cache[i] + (updated - (double)previous));
     }
-    __syncthreads();
-    for (int32_t i = threadIdx.x; i < count; i += kJoinThreads) {
-        const int64_t ri = slot[i];
+    if (__any(inexact) && threadIdx.x % 64 == 0) atomicOr(s.inexact, 1);
+}
+
+// Writes the joined OTU's row and column, moves the OTU at `other` (index 0,
+// or the last) into y's place (y leaves; `other` is dropped), and sums the
+// joined OTU's row in index order on wave 0 of workgroup 0.
+__global__ __launch_bounds__(kStepThreads) void join_b_kernel(NjState s, int32_t count, int32_t base, uint32_t parent,
+                                                              int32_t variant)
+{
+    __shared__ __attribute__((aligned(16))) float stage[kStage];
+    const int64_t n = s.n;
+    float* mb = s.m + (int64_t)base * n + base;
+    uint32_t* map = s.map + base;
+    float* cache = s.cache + base;
+    const int32_t x = s.pick[0], y = s.pick[1];
+    const int32_t other = variant == MSA_NJ_SYMMETRIC ? 0 : count - 1;
+    const bool move = y != other;
+    // Row and column x; y's own row and column are rewritten below (or y is
+    // dropped), so each element has one writer.
+    for (int32_t i = blockIdx.x * kStepThreads + threadIdx.x; i < count; i += gridDim.x * kStepThreads) {
+        if (i == y) continue;
         const float v = (float)s.upd[i];
-        s.m[ri * n + rx] = v;
-        s.m[rx * n + ri] = v;
+        mb[(int64_t)x * n + i] = v;
+        mb[(int64_t)i * n + x] = v;
     }
-    // The joined OTU's row sum: float += double, in index order.
+    if (move) {
+        // The new row y is `other`'s row after this join: the joined OTU's
+        // distance where `other` meets x, else unchanged.
+        for (int32_t k = blockIdx.x * kStepThreads + threadIdx.x; k < count; k += gridDim.x * kStepThreads) {
+            if (k == y || k == other) continue;
+            const float v = other == x ? (float)s.upd[k] : (k == x ? (float)s.upd[other] : mb[(int64_t)other * n + k]);
+            mb[(int64_t)y * n + k] = v;
+            mb[(int64_t)k * n + y] = v;
+        }
+    }
+    if (blockIdx.x != 0) return;
+    // The joined OTU's row sum, in index order: float += double.  When every
+    // term is a float (always, unless a halving underflowed), that is the
+    // float sum, because rounding the exact double sum of two floats to
+    // float is the float sum (53 >= 2 * 24 + 2 bits).  The terms are staged
+    // in LDS and lane 0 reads 64 at a time into registers, so the dependent
+    // chain is one add per term.
+    const bool inexact = *s.inexact != 0;
     float new_sum = 0.0f;
-    for (int32_t c0 = 0; c0 < count; c0 += kSumChunk) {
-        const int32_t len = count - c0 < kSumChunk ? count - c0 : kSumChunk;
+    for (int32_t c0 = 0; c0 < count; c0 += kStage) {
+        const int32_t len = count - c0 < kStage ? count - c0 : kStage;
         __syncthreads();
-        for (int32_t i = threadIdx.x; i < len; i += kJoinThreads) stage[i] = s.upd[c0 + i];
+        if (!inexact)
+            for (int32_t i = threadIdx.x; i < len; i += kStepThreads) stage[i] = (float)s.upd[c0 + i];
         __syncthreads();
-        if (threadIdx.x == 0)
-            for (int32_t i = 0; i < len; ++i) new_sum = (float)((double)new_sum + stage[i]);
+        if (threadIdx.x != 0) continue;
+        if (inexact) {
+            for (int32_t i = 0; i < len; ++i) new_sum = (float)((double)new_sum + s.upd[c0 + i]);
+            continue;
+        }
+        int32_t i = 0;
+        for (; i + 64 <= len; i += 64) {
+            float4 v[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) v[q] = *reinterpret_cast<const float4*>(&stage[i + 4 * q]);
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                new_sum += v[q].x;
+                new_sum += v[q].y;
+                new_sum += v[q].z;
+                new_sum += v[q].w;
+            }
+        }
+        for (; i < len; ++i) new_sum += stage[i];
     }
     if (threadIdx.x == 0) {
         cache[x] = new_sum;
         map[x] = parent;
-        const int32_t other = variant == MSA_NJ_SYMMETRIC ? 0 : count - 1;
-        if (y != other) {
-            const float tc = cache[other];
-            cache[other] = cache[y];
-            cache[y] = tc;
-            const int32_t ts = slot[other];
-            slot[other] = slot[y];
-            slot[y] = ts;
-            const uint32_t tm = map[other];
-            map[other] = map[y];
-            map[y] = tm;
+        if (move) {
+            cache[y] = other == x ? new_sum : cache[other];
+            map[y] = other == x ? parent : map[other];
         }
-        *s.best = 0;
+        *s.inexact = 0;          // for the next join (join_a only ever sets it)
     }
 }
 
-// One workgroup per row, up to 8 per compute unit (rows are taken round-robin).
-int argmax_grid(int32_t count, int cus) { return std::max(1, std::min(count - 1, cus * 8)); }
+// Arg-max workgroups: one per row, up to 8 per compute unit.
+int step_grid(int32_t count, int cus) { return std::max(1, std::min({count - 1, cus * 8, kMaxStepBlocks})); }
+
+// Workgroups for the per-OTU phases of a join.
+int otu_grid(int32_t count) { return std::max(1, (count + kStepThreads - 1) / kStepThreads); }
 
 struct NjBuffers {
     void* block = nullptr;
@@ -250,21 +321,22 @@ int nj_alloc(int32_t n, NjBuffers& b)
     auto up = [](size_t v) { return (v + 255) & ~size_t(255); };
     const size_t nn = (size_t)n * (size_t)n;
     const size_t o_m = 0;
-    const size_t o_slot = up(o_m + nn * sizeof(float));
-    const size_t o_map = up(o_slot + (size_t)n * sizeof(int32_t));
+    const size_t o_map = up(o_m + nn * sizeof(float));
     const size_t o_cache = up(o_map + (size_t)n * sizeof(uint32_t));
     const size_t o_upd = up(o_cache + (size_t)n * sizeof(float));
-    const size_t o_best = up(o_upd + (size_t)n * sizeof(double));
-    const size_t o_error = up(o_best + sizeof(unsigned long long));
+    const size_t o_partial = up(o_upd + (size_t)n * sizeof(double));
+    const size_t o_pick = up(o_partial + kMaxStepBlocks * sizeof(unsigned long long));
+    const size_t o_error = up(o_pick + 4 * sizeof(int32_t));
     const size_t bytes = up(o_error + sizeof(int32_t));
     MSA_HIP(hipMalloc(&b.block, bytes));
     char* base = static_cast<char*>(b.block);
     b.s.m = reinterpret_cast<float*>(base + o_m);
-    b.s.slot = reinterpret_cast<int32_t*>(base + o_slot);
     b.s.map = reinterpret_cast<uint32_t*>(base + o_map);
     b.s.cache = reinterpret_cast<float*>(base + o_cache);
     b.s.upd = reinterpret_cast<double*>(base + o_upd);
-    b.s.best = reinterpret_cast<unsigned long long*>(base + o_best);
+    b.s.partial = reinterpret_cast<unsigned long long*>(base + o_partial);
+    b.s.pick = reinterpret_cast<int32_t*>(base + o_pick);
+    b.s.inexact = b.s.pick + 2;
     b.s.error = reinterpret_cast<int32_t*>(base + o_error);
     b.s.n = n;
     return MSA_OK;
@@ -291,9 +363,11 @@ int nj_run(const T* d_packed, int32_t n, int32_t variant, msa_nj_node* d_nodes, 
     for (int32_t it = 0; it + 1 < n; ++it) {
         const int32_t count = n - it;
         const int32_t base = variant == MSA_NJ_SYMMETRIC ? it : 0;
-        hipLaunchKernelGGL(argmax_kernel, dim3(argmax_grid(count, cus)), dim3(kArgmaxThreads), 0, stream, b.s, count,
-                           base);
-        hipLaunchKernelGGL(join_kernel, dim3(1), dim3(kJoinThreads), 0, stream, b.s, count, base,
+        const int grid = step_grid(count, cus);
+        hipLaunchKernelGGL(argmax_kernel, dim3(grid), dim3(kStepThreads), 0, stream, b.s, count, base);
+        hipLaunchKernelGGL(join_a_kernel, dim3(otu_grid(count)), dim3(kStepThreads), 0, stream, b.s, count, base,
+                           (uint32_t)(n + it), grid);
+        hipLaunchKernelGGL(join_b_kernel, dim3(otu_grid(count)), dim3(kStepThreads), 0, stream, b.s, count, base,
                            (uint32_t)(n + it), variant);
     }
     MSA_HIP(hipGetLastError());

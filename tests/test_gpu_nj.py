@@ -38,7 +38,7 @@ def assert_same_tree(got, exp):
 
 
 @pytest.mark.parametrize("variant", [N.NJ_SYMMETRIC, N.NJ_LINEAR])
-@pytest.mark.parametrize("n", [2, 3, 4, 5, 17, 64, 250])
+@pytest.mark.parametrize("n", [2, 3, 4, 5, 17, 64, 250, 700, 1500])
 def test_random_matrices(n, variant):
     rng = np.random.default_rng(n * 10 + variant)
     d = rng.normal(0, 100, n * (n - 1) // 2).astype(np.float32)
