@@ -382,6 +382,121 @@ namespace museqa
         }
     }
 
+    // ---- phylogeny --------------------------------------------------------
+
+    namespace phylogeny
+    {
+        const otu& guidetree::operator[](oturef ref) const
+        {
+            if (ref >= m_nodes.size()) throw exception("guide tree node out of range");
+            return m_nodes[ref];
+        }
+
+        const otu& guidetree::root() const
+        {
+            if (m_nodes.empty()) throw exception("the guide tree is empty");
+            return m_nodes.back();
+        }
+
+        namespace
+        {
+            static_assert(sizeof(otu) == sizeof(msa_nj_node), "otu mirrors msa_nj_node");
+
+            // Neighbor joining in the HIP kernels behind the C ABI: the
+            // reference's sequential semantics (impl/sequential.cpp), either
+            // matrix-removal variant.
+            struct hip_njoining : public algorithm
+            {
+                int variant;
+                explicit hip_njoining(int v) : variant {v} {}
+
+                guidetree run(const context& ctx) const override
+                {
+                    const size_t n = ctx.count;
+                    if (n > ctx.matrix.count())
+                        throw exception("the distance matrix holds fewer sequences than requested");
+                    if (n > 65535) throw exception("at most 65535 sequences can be joined");
+                    if (n == 0) return guidetree {};
+                    std::vector<otu> nodes(2 * n - 1);
+                    if (n == 1) {
+                        nodes[0] = otu {undefined, {undefined, undefined}, 3.40282347e+38f, 0};
+                        return guidetree {std::move(nodes), 1};
+                    }
+                    // The matrix of the first n sequences is the first n(n-1)/2
+                    // entries of the packed buffer (pair order c = i(i-1)/2 + j).
+                    int devices = 0;
+                    check(msa_device_count(&devices));
+                    if (devices < 1) throw exception("no GPU available for neighbor joining");
+                    check(msa_nj_build(ctx.matrix.raw().data(), int32_t(n), variant,
+                                       reinterpret_cast<msa_nj_node*>(nodes.data())));
+                    return guidetree {std::move(nodes), uint32_t(n)};
+                }
+            };
+
+            using factory = std::function<std::unique_ptr<algorithm>()>;
+
+            template <int V>
+            std::unique_ptr<algorithm> make_nj() { return std::unique_ptr<algorithm>(new hip_njoining(V)); }
+
+            // src/phylogeny/phylogeny.cpp:26-41.  The reference's hybrid and
+            // sequential variants share one semantics here (the sequential
+            // one's pick order, float for float), so each name maps to its
+            // matrix-removal variant.
+            const std::vector<std::pair<std::string, factory>>& registry()
+            {
+                static const std::vector<std::pair<std::string, factory>> r = {
+                    {"default", make_nj<MSA_NJ_SYMMETRIC>},
+                    {"njoining", make_nj<MSA_NJ_SYMMETRIC>},
+                    {"hybrid", make_nj<MSA_NJ_SYMMETRIC>},
+                    {"linear", make_nj<MSA_NJ_LINEAR>},
+                    {"symmetric", make_nj<MSA_NJ_SYMMETRIC>},
+                    {"njoining-hybrid", make_nj<MSA_NJ_SYMMETRIC>},
+                    {"njoining-linear", make_nj<MSA_NJ_LINEAR>},
+                    {"njoining-symmetric", make_nj<MSA_NJ_SYMMETRIC>},
+                    {"sequential", make_nj<MSA_NJ_SYMMETRIC>},
+                    {"distributed", make_nj<MSA_NJ_SYMMETRIC>},
+                    {"njoining-sequential", make_nj<MSA_NJ_SYMMETRIC>},
+                    {"njoining-sequential-linear", make_nj<MSA_NJ_LINEAR>},
+                    {"njoining-distributed", make_nj<MSA_NJ_SYMMETRIC>},
+                    {"njoining-distributed-linear", make_nj<MSA_NJ_LINEAR>},
+                    {"hip", make_nj<MSA_NJ_SYMMETRIC>},
+                    {"hip-linear", make_nj<MSA_NJ_LINEAR>},
+                };
+                return r;
+            }
+        }
+
+        bool algorithm::has(const std::string& name)
+        {
+            for (const auto& e : registry())
+                if (e.first == name) return true;
+            return false;
+        }
+
+        std::unique_ptr<algorithm> algorithm::make(const std::string& name)
+        {
+            for (const auto& e : registry())
+                if (e.first == name) return e.second();
+            raise("unknown phylogeny algorithm '%s'", name);
+        }
+
+        const std::vector<std::string>& algorithm::list() noexcept
+        {
+            static const std::vector<std::string> names = [] {
+                std::vector<std::string> v;
+                for (const auto& e : registry()) v.push_back(e.first);
+                return v;
+            }();
+            return names;
+        }
+
+        guidetree run(const pairwise::distance_matrix& matrix, size_t count, const std::string& name)
+        {
+            const auto worker = algorithm::make(name);
+            return worker->run({matrix, count});
+        }
+    }
+
     // ---- module::pairwise (src/pairwise.cpp:27-55) ------------------------
 
     namespace
@@ -407,5 +522,20 @@ namespace museqa
         const auto table = museqa::pairwise::scoring_table::make(option(opts, "scoring-table", "default"));
         auto result = museqa::pairwise::run(db, table, option(opts, "pairwise", "default"));
         return conduit {db, std::move(result), db.count()};
+    }
+
+    // ---- module::phylogeny (src/phylogeny.cpp:25-47) ----------------------
+
+    bool module::phylogeny::check(const options& opts)
+    {
+        const auto algoname = option(opts, "phylogeny", "default");
+        if (!museqa::phylogeny::algorithm::has(algoname)) raise("unknown phylogeny algorithm chosen: '%s'", algoname);
+        return true;
+    }
+
+    module::phylogeny::conduit module::phylogeny::run(const options& opts, const pairwise::conduit& previous)
+    {
+        auto tree = museqa::phylogeny::run(previous.distances, previous.count, option(opts, "phylogeny", "default"));
+        return conduit {previous.db, std::move(tree)};
     }
 }

@@ -13,7 +13,12 @@
 //                                src/pairwise/table.cu:254-330
 //   pairwise::database           src/pairwise/database.cuh:27-91, database.cu:24-72
 //   module::pairwise::{run, check}  src/pairwise.cpp:27-55
-// The algorithm registry holds the reference's names plus the GPU variants;
+//   phylogeny::{otu, guidetree, context, algorithm, run}
+//                                src/phylogeny/phylogeny.cuh:24-134,
+//                                src/phylogeny/phylogeny.cpp:26-75,
+//                                src/phylogeny/tree.cuh
+//   module::phylogeny::{run, check} src/phylogeny.cpp:25-47
+// The algorithm registries hold the reference's names plus the GPU variants;
 // every name runs on the GPU (algorithm::list() gives them).
 #pragma once
 
@@ -202,6 +207,62 @@ namespace museqa
         inline size_t nchoose(size_t n) noexcept { return n ? n * (n - 1) / 2 : 0; }
     }
 
+    namespace phylogeny
+    {
+        using oturef = uint32_t;
+
+        // One node of the guide tree (src/phylogeny/tree.cuh, star.cuh:63-80):
+        // leaves are 0..count-1, joins are numbered count, count+1, ... in
+        // join order and the root is the last node.
+        struct otu
+        {
+            oturef parent;
+            oturef child[2];                // ordered by id; undefined for a leaf
+            float distance;                 // branch length to the parent; FLT_MAX for the root
+            uint32_t level;                 // 0 for a leaf, 1 + max(children) above
+        };
+
+        enum : oturef { undefined = 0xFFFFFFFFu };
+
+        class guidetree
+        {
+            public:
+                guidetree() = default;
+                guidetree(std::vector<otu> nodes, uint32_t leaves) : m_nodes {std::move(nodes)}, m_leaves {leaves} {}
+
+                const otu& operator[](oturef ref) const;
+                const otu& root() const;
+                uint32_t leaves() const noexcept { return m_leaves; }
+                size_t count() const noexcept { return m_nodes.size(); }
+                const std::vector<otu>& raw() const noexcept { return m_nodes; }
+
+            private:
+                std::vector<otu> m_nodes;
+                uint32_t m_leaves = 0;
+        };
+
+        // src/phylogeny/phylogeny.cuh:77-112
+        struct context
+        {
+            const pairwise::distance_matrix& matrix;
+            const size_t count;
+        };
+
+        struct algorithm
+        {
+            virtual ~algorithm() = default;
+            virtual guidetree run(const context&) const = 0;
+
+            static bool has(const std::string&);
+            static std::unique_ptr<algorithm> make(const std::string&);
+            static const std::vector<std::string>& list() noexcept;
+        };
+
+        // src/phylogeny/phylogeny.cuh:114-134
+        guidetree run(const pairwise::distance_matrix& matrix, size_t count,
+                      const std::string& algorithm = "default");
+    }
+
     namespace module
     {
         // The pipeline module contract: option values in, conduit out.
@@ -218,6 +279,21 @@ namespace museqa
             static const char* name() { return "pairwise"; }
             static bool check(const options& opts);
             static conduit run(const options& opts, const museqa::database& db);
+        };
+
+        // src/phylogeny.cpp:25-47: the pairwise conduit in, the tree out.
+        struct phylogeny
+        {
+            struct conduit
+            {
+                museqa::database db;
+                museqa::phylogeny::guidetree tree;
+            };
+
+            using options = std::map<std::string, std::string>;
+            static const char* name() { return "phylogeny"; }
+            static bool check(const options& opts);
+            static conduit run(const options& opts, const pairwise::conduit& previous);
         };
     }
 }

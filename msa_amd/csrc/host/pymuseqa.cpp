@@ -84,4 +84,36 @@ PYBIND11_MODULE(_museqa, m)
         auto c = module::pairwise::run(opts, db);
         return py::make_tuple(c.db, c.distances, c.count);
     });
+
+    // museqa::phylogeny (src/phylogeny/phylogeny.cuh); the reference ships no
+    // Python binding for it, this one follows the pairwise binding's shape.
+    py::class_<phylogeny::guidetree>(m, "GuideTree")
+        .def("node", [](const phylogeny::guidetree& t, phylogeny::oturef ref) {
+            const auto& v = t[ref];
+            return py::make_tuple(v.parent, py::make_tuple(v.child[0], v.child[1]), v.distance, v.level);
+        })
+        .def_property_readonly("leaves", &phylogeny::guidetree::leaves)
+        .def_property_readonly("count", &phylogeny::guidetree::count)
+        .def("root", [](const phylogeny::guidetree& t) {
+            t.root();
+            return phylogeny::oturef(t.count() - 1);
+        })
+        .def("raw", [](const phylogeny::guidetree& t) {
+            // (parent, child0, child1, distance bits, level) per node, as bytes
+            // with the layout of msa_nj_node.
+            return py::bytes(reinterpret_cast<const char*>(t.raw().data()), t.raw().size() * sizeof(phylogeny::otu));
+        });
+
+    m.def("phylogeny_run", [](const pairwise::distance_matrix& d, size_t count, const std::string& algorithm) {
+        py::gil_scoped_release release;
+        return phylogeny::run(d, count, algorithm);
+    });
+    m.def("phylogeny_algorithms", &phylogeny::algorithm::list);
+    m.def("phylogeny_has_algorithm", &phylogeny::algorithm::has);
+    m.def("phylogeny_module_check", &module::phylogeny::check);
+    m.def("phylogeny_module_run", [](const module::phylogeny::options& opts, const database& db,
+                                     const pairwise::distance_matrix& d, size_t count) {
+        auto c = module::phylogeny::run(opts, module::pairwise::conduit {db, d, count});
+        return py::make_tuple(c.db, c.tree);
+    });
 }

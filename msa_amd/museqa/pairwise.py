@@ -45,6 +45,10 @@ class DistanceMatrix:
     def scores(self):
         return _list(self._d.raw())
 
+    @property
+    def native(self):
+        return self._d
+
 
 def list():  # noqa: A001 -- the reference's name
     return _list(core.algorithms())

@@ -57,3 +57,15 @@ def random_db(rng, n, lo, hi, dirty=False, alphabet=24):
 
 def strings_db(strings):
     return pack_blocks([M.encode(s) for s in strings])
+
+
+def assert_same_tree(got, exp):
+    assert len(got) == len(exp)
+    for field in ("parent", "child", "level"):
+        bad = np.nonzero(np.any((got[field] != exp[field]).reshape(len(got), -1), axis=1))[0]
+        assert len(bad) == 0, "%s differs at nodes %s: gpu %s oracle %s" % (
+            field, bad[:5], got[field][bad[:5]], exp[field][bad[:5]])
+    g, e = got["distance"], exp["distance"]
+    same = (g.view(np.uint32) == e.view(np.uint32)) | (np.isnan(g) & np.isnan(e))
+    bad = np.nonzero(~same)[0]
+    assert len(bad) == 0, "distance differs at nodes %s: gpu %s oracle %s" % (bad[:5], g[bad[:5]], e[bad[:5]])

@@ -25,24 +25,12 @@ def device():
     N.check(N.lib().msa_device_select(0))
 
 
-def assert_same_tree(got, exp):
-    assert len(got) == len(exp)
-    for field in ("parent", "child", "level"):
-        bad = np.nonzero(np.any((got[field] != exp[field]).reshape(len(got), -1), axis=1))[0]
-        assert len(bad) == 0, "%s differs at nodes %s: gpu %s oracle %s" % (
-            field, bad[:5], got[field][bad[:5]], exp[field][bad[:5]])
-    g, e = got["distance"], exp["distance"]
-    same = (g.view(np.uint32) == e.view(np.uint32)) | (np.isnan(g) & np.isnan(e))
-    bad = np.nonzero(~same)[0]
-    assert len(bad) == 0, "distance differs at nodes %s: gpu %s oracle %s" % (bad[:5], g[bad[:5]], e[bad[:5]])
-
-
 @pytest.mark.parametrize("variant", [N.NJ_SYMMETRIC, N.NJ_LINEAR])
 @pytest.mark.parametrize("n", [2, 3, 4, 5, 17, 64, 250, 700, 1500])
 def test_random_matrices(n, variant):
     rng = np.random.default_rng(n * 10 + variant)
     d = rng.normal(0, 100, n * (n - 1) // 2).astype(np.float32)
-    assert_same_tree(N.nj_build(d, n, variant), ORACLE.build(d, n, variant))
+    H.assert_same_tree(N.nj_build(d, n, variant), ORACLE.build(d, n, variant))
 
 
 @pytest.mark.parametrize("variant", [N.NJ_SYMMETRIC, N.NJ_LINEAR])
@@ -50,7 +38,7 @@ def test_ties(variant):
     """All distances equal: every Q ties at every iteration."""
     for n in (3, 7, 40):
         d = np.ones(n * (n - 1) // 2, dtype=np.float32)
-        assert_same_tree(N.nj_build(d, n, variant), ORACLE.build(d, n, variant))
+        H.assert_same_tree(N.nj_build(d, n, variant), ORACLE.build(d, n, variant))
 
 
 @pytest.mark.parametrize("variant", [N.NJ_SYMMETRIC, N.NJ_LINEAR])
@@ -58,7 +46,7 @@ def test_additive_trees(variant):
     for n, seed in [(6, 1), (33, 2), (120, 3)]:
         packed, splits, _ = M.additive_tree(np.random.default_rng(seed), n)
         got = N.nj_build(-packed, n, variant)
-        assert_same_tree(got, ORACLE.build(-packed, n, variant))
+        H.assert_same_tree(got, ORACLE.build(-packed, n, variant))
         assert M.nj_splits(got, n) == splits
 
 
@@ -66,7 +54,7 @@ def test_small_inputs():
     assert len(N.nj_build(np.zeros(0, dtype=np.float32), 0)) == 0
     assert len(N.nj_build(np.zeros(0, dtype=np.float32), 1)) == 0
     got = N.nj_build(np.array([7.0], dtype=np.float32), 2)
-    assert_same_tree(got, ORACLE.build(np.array([7.0], dtype=np.float32), 2))
+    H.assert_same_tree(got, ORACLE.build(np.array([7.0], dtype=np.float32), 2))
 
 
 @pytest.mark.parametrize("variant", [N.NJ_SYMMETRIC, N.NJ_LINEAR])
@@ -88,8 +76,8 @@ def test_pairwise_scores_chained_on_device(variant):
     got = d_nodes.download(N.NJ_NODE, 2 * n - 1)
     scores = d_scores.download(np.int32, npairs).astype(np.float32)
     exp = ORACLE.build(scores, n, variant)
-    assert_same_tree(got, exp)
-    assert_same_tree(N.nj_build(scores, n, variant), exp)
+    H.assert_same_tree(got, exp)
+    H.assert_same_tree(N.nj_build(scores, n, variant), exp)
 
 
 def test_uniform_batch_512_tree():
@@ -98,4 +86,4 @@ def test_uniform_batch_512_tree():
     w = W.uniform_batch(H.tables())
     scores = N.align(w.blocks, w.offsets, w.sizes, w.table, w.penalty).astype(np.float32)
     for variant in (N.NJ_SYMMETRIC, N.NJ_LINEAR):
-        assert_same_tree(N.nj_build(scores, w.nseq, variant), ORACLE.build(scores, w.nseq, variant))
+        H.assert_same_tree(N.nj_build(scores, w.nseq, variant), ORACLE.build(scores, w.nseq, variant))

@@ -221,3 +221,67 @@ def test_module_pairwise_contract():
     out_db, dm, count = core.module_run(opts, db.native)
     assert count == 5 and dm.count == 5
     assert [int(v) for v in dm.raw()] == H.golden_scores()["mock3.fasta/blosum62"]["scores"]
+
+
+# ---- phylogeny surface ---------------------------------------------------------
+
+PHYLOGENY_NAMES = {
+    # src/phylogeny/phylogeny.cpp:26-41: name -> matrix-removal variant
+    "default": 0, "njoining": 0, "hybrid": 0, "linear": 1, "symmetric": 0, "njoining-hybrid": 0,
+    "njoining-linear": 1, "njoining-symmetric": 0, "sequential": 0, "distributed": 0,
+    "njoining-sequential": 0, "njoining-sequential-linear": 1, "njoining-distributed": 0,
+    "njoining-distributed-linear": 1, "hip": 0, "hip-linear": 1,
+}
+
+
+def test_phylogeny_registry():
+    from msa_amd.museqa import phylogeny as mpg
+    from msa_amd.museqa._core import core
+    assert sorted(mpg.list()) == sorted(PHYLOGENY_NAMES)
+    with pytest.raises(RuntimeError, match="unknown phylogeny algorithm 'nope'"):
+        mpg.run(mpw.DistanceMatrix([1.0], 2), algorithm="nope")
+    assert core.phylogeny_module_check({"phylogeny": "njoining-linear"})
+    with pytest.raises(RuntimeError, match="unknown phylogeny algorithm chosen: 'zzz'"):
+        core.phylogeny_module_check({"phylogeny": "zzz"})
+
+
+def test_phylogeny_trivial_counts():
+    """No join happens below two sequences: no tree at 0, a lone leaf at 1."""
+    from msa_amd.museqa import phylogeny as mpg
+    d = mpw.DistanceMatrix([4.0, 5.0, 6.0], 3)
+    assert len(mpg.run(d, count=0)) == 0
+    one = mpg.run(d, count=1)
+    assert len(one) == 1 and one.leaves == 1 and one.root == 0
+    assert one[0]["parent"] == mpg.UNDEFINED and one[0]["child"] == (mpg.UNDEFINED, mpg.UNDEFINED)
+    with pytest.raises(RuntimeError, match="fewer sequences"):
+        mpg.run(d, count=4)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(PHYLOGENY_NAMES))
+def test_phylogeny_algorithms_match_oracle(name):
+    from msa_amd.museqa import phylogeny as mpg
+    rng = np.random.default_rng(11)
+    n = 57
+    scores = rng.integers(-500, 500, n * (n - 1) // 2).astype(np.float32)
+    tree = mpg.run(mpw.DistanceMatrix(scores.tolist(), n), algorithm=name)
+    want = M.NjOracle().build(scores, n, PHYLOGENY_NAMES[name])
+    assert tree.leaves == n and tree.root == 2 * n - 2
+    H.assert_same_tree(tree.nodes(), want)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fixture", ["mock1.fasta", "mock2.fasta", "mock3.fasta"])
+def test_module_pipeline_pairwise_then_phylogeny(fixture):
+    """module::pairwise's conduit into module::phylogeny (src/phylogeny.cpp:25-34):
+    the tree is the oracle's over the golden scores."""
+    from msa_amd.museqa import phylogeny as mpg
+    from msa_amd.museqa._core import core
+    db = Database.load(os.path.join(H.GOLDEN, fixture))
+    out_db, dm, count = core.module_run({"scoring-table": "blosum62"}, db.native)
+    for name, variant in [("default", 0), ("njoining-linear", 1)]:
+        tdb, tree = core.phylogeny_module_run({"phylogeny": name}, out_db, dm, count)
+        assert tdb.count == db.count and tree.count == max(0, 2 * count - 1)
+        golden = np.array(H.golden_scores()["%s/blosum62" % fixture]["scores"], dtype=np.float32)
+        want = M.NjOracle().build(golden, count, variant)
+        H.assert_same_tree(mpg.GuideTree(tree).nodes(), want)
