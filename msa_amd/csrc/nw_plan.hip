@@ -149,37 +149,89 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
     const int64_t long_threshold = kernel == MSA_KERNEL_LONG ? 0
                                  : (opt.long_threshold > 0 ? opt.long_threshold : tuning->long_threshold);
     int64_t cells = 0;
-    auto visit = [&](int32_t first, int32_t second, int64_t idx) -> int {
-        if (first < 0 || first >= nseq || second < 0 || second >= nseq)
-            return fail(MSA_ERR_INVALID_ARGUMENT, "pair %lld names sequence outside 0..%d", (long long)idx, nseq - 1);
+    // Targets of one query are queued longest first: sequences ranked by
+    // (rows descending, id descending).
+    std::vector<int32_t> by_len(nseq), rank(nseq);
+    for (int32_t q = 0; q < nseq; ++q) by_len[q] = q;
+    std::sort(by_len.begin(), by_len.end(), [&](int32_t a, int32_t b) {
+        return seqs[a].first_end != seqs[b].first_end ? seqs[a].first_end > seqs[b].first_end : a > b;
+    });
+    for (int32_t r = 0; r < nseq; ++r) rank[by_len[r]] = r;
+    // Routing of one pair (first, second) as the reference orders it.
+    enum { kFast, kLong, kPlain };
+    struct Route { int32_t kind, qkey, target; int64_t cells; };
+    auto route = [&](int32_t first, int32_t second) -> Route {
         const int32_t rowref = seqs[first].blocks > seqs[second].blocks ? first : second;
         const int32_t colref = rowref == first ? second : first;
         const HostSeq& r = seqs[rowref];
         const HostSeq& col = seqs[colref];
-        cells += (int64_t)r.first_end * col.last_real;
+        const int64_t c = (int64_t)r.first_end * col.last_real;
         const bool clean = col.first_end == col.last_real && r.first_end > 0 && col.last_real > 0;
-        if (kernel == MSA_KERNEL_PLAIN || !clean) {
-            hp.plain_pairs.push_back(first);
-            hp.plain_pairs.push_back(second);
-            hp.plain_index.push_back((int32_t)idx);
-            return MSA_OK;
-        }
+        if (kernel == MSA_KERNEL_PLAIN || !clean) return {kPlain, 0, 0, c};
         const bool row_query = r.first_end < col.last_real || (r.first_end == col.last_real && rowref < colref);
         const int32_t q = row_query ? rowref : colref;
         const int32_t t = row_query ? colref : rowref;
         const int32_t transpose = (!symmetric && !row_query) ? 1 : 0;
-        if (kernel != MSA_KERNEL_BATCHED && std::min(r.first_end, col.last_real) >= long_threshold)
-            longs.push_back({q * 2 + transpose, t, (int32_t)idx});
-        else
-            fast.push_back({q * 2 + transpose, t, (int32_t)idx});
-        return MSA_OK;
+        const bool is_long = kernel != MSA_KERNEL_BATCHED && std::min(r.first_end, col.last_real) >= long_threshold;
+        return {is_long ? kLong : kFast, q * 2 + transpose, t, c};
+    };
+    auto add_plain = [&](int32_t first, int32_t second, int64_t idx) {
+        hp.plain_pairs.push_back(first);
+        hp.plain_pairs.push_back(second);
+        hp.plain_index.push_back((int32_t)idx);
     };
     if (pairs) {
-        for (int64_t c = 0; c < npairs; ++c) MSA_TRY(visit(pairs[2 * c], pairs[2 * c + 1], c));
+        for (int64_t c = 0; c < npairs; ++c) {
+            const int32_t first = pairs[2 * c], second = pairs[2 * c + 1];
+            if (first < 0 || first >= nseq || second < 0 || second >= nseq)
+                return fail(MSA_ERR_INVALID_ARGUMENT, "pair %lld names sequence outside 0..%d", (long long)c,
+                            nseq - 1);
+            const Route rt = route(first, second);
+            cells += rt.cells;
+            if (rt.kind == kPlain) add_plain(first, second, c);
+            else (rt.kind == kLong ? longs : fast).push_back({rt.qkey, rt.target, (int32_t)c});
+        }
+        // Group the batched pairs by query key, targets in rank order, list
+        // order within a target: two stable counting passes.
+        if (!fast.empty()) {
+            std::vector<Fast> tmp(fast.size());
+            std::vector<int64_t> start((size_t)2 * nseq + 1, 0);
+            for (const Fast& f : fast) ++start[rank[f.target] + 1];
+            for (int32_t r = 0; r < nseq; ++r) start[r + 1] += start[r];
+            for (const Fast& f : fast) tmp[start[rank[f.target]]++] = f;
+            std::fill(start.begin(), start.end(), 0);
+            for (const Fast& f : tmp) ++start[f.query + 1];
+            for (int32_t k = 0; k < 2 * nseq; ++k) start[k + 1] += start[k];
+            for (const Fast& f : tmp) fast[start[f.query]++] = f;
+        }
     } else {
-        int64_t c = 0;
-        for (int32_t i = 1; i < nseq; ++i)
-            for (int32_t j = 0; j < i; ++j, ++c) MSA_TRY(visit(i, j, c));
+        // All pairs, generated query by query with each query's partners in
+        // rank order, so the batched list comes out grouped and ordered with
+        // no sort.  Every pair is met from both of its sequences: batched and
+        // long pairs are taken on their query's side, plain pairs on the side
+        // of their larger index.
+        std::vector<Fast> transposed;
+        for (int32_t q = 0; q < nseq; ++q) {
+            transposed.clear();
+            for (int32_t k = 0; k < nseq; ++k) {
+                const int32_t t = by_len[k];
+                if (t == q) continue;
+                const int32_t first = std::max(q, t), second = std::min(q, t);
+                const int64_t idx = (int64_t)first * (first - 1) / 2 + second;
+                const Route rt = route(first, second);
+                if (rt.kind == kPlain) {
+                    if (q == first) cells += rt.cells, add_plain(first, second, idx);
+                    continue;
+                }
+                if (rt.qkey >> 1 != q) continue;
+                cells += rt.cells;
+                const Fast f{rt.qkey, rt.target, (int32_t)idx};
+                if (rt.kind == kLong) longs.push_back(f);
+                else if (rt.qkey & 1) transposed.push_back(f);
+                else fast.push_back(f);
+            }
+            fast.insert(fast.end(), transposed.begin(), transposed.end());
+        }
     }
     hp.info.pairs = npairs;
     hp.info.cells = cells;
@@ -211,12 +263,6 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
     // length, cut into items of up to 16 targets, queue the items costliest
     // first.  Each item carries the tile shape of its query's length bin.
     if (!fast.empty()) {
-        std::sort(fast.begin(), fast.end(), [&](const Fast& x, const Fast& y) {
-            if (x.query != y.query) return x.query < y.query;
-            const int32_t lx = seqs[x.target].first_end, ly = seqs[y.target].first_end;
-            if (lx != ly) return lx > ly;
-            return x.index < y.index;
-        });
         // Targets per item: 16 (four rounds of four) when the plan has enough
         // pairs to give every resident wave four items, else 8 or 4, so that
         // small plans (serving jobs) still spread over the whole GPU.
