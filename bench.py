@@ -299,6 +299,95 @@ def phylogeny_run(N, w, steps, warmup, cpu=True, variant=0):
     return result
 
 
+def fasta_run(N, steps, warmup, target_mib, cpu=True, cpu_mib=64):
+    """FASTA loading on the GPU (msa_fasta_parse_device) over a protein
+    database already in device memory: parse, encode and pack into the
+    sequence store, every step.  The store is compared with the oracle's
+    (oracle/fasta_oracle.c) over the whole text.  CPU baseline: the
+    reference's own parser (oracle/_ref/libref_fasta.so: fasta.cpp +
+    encoder.cpp + database.cpp, one thread, reading the file from the page
+    cache) on the first `cpu_mib` MiB of the same text, cut at a record."""
+    import tempfile
+    from msa_amd import workloads as W
+    from oracle import museqa_ref as M
+    text, desc = W.fasta_database(target_bytes=int(target_mib) << 20)
+    n = len(text)
+    log("fasta workload:", desc, "bytes", n)
+    d_text = N.DeviceBuffer(n)
+    d_text.upload(text)
+    ld = N.FastaLoader()
+    for _ in range(warmup):
+        ld.parse_device(d_text.ptr, n)
+    stages = {"total_ms": 0.0, "lines_ms": 0.0, "records_ms": 0.0, "emit_ms": 0.0}
+    t0 = time.time()
+    for _ in range(steps):
+        ld.parse_device(d_text.ptr, n)
+        for k, v in ld.timing().items():
+            stages[k] += v / steps
+    wall_ms = 1e3 * (time.time() - t0) / steps
+    info = ld.info()
+    got = ld.store()
+    exp = M.FastaOracle().parse(text.tobytes())
+    same = all(np.array_equal(g, e) for g, e in zip(got, exp))
+    out_bytes = 2 * info["blocks"] + 28 * info["sequences"]      # blocks + offset, size, description range
+    emit_bytes = n + 2 * info["blocks"]
+    result = {
+        "metric": "FASTA load (parse + encode + pack into the sequence store) on 1 MI355X",
+        "value": round(n / (wall_ms * 1e-3) / 1e9, 2), "unit": "GB/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": round(wall_ms, 3), "higher_is_better": True, "scaling": "strong",
+        "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "config": {"workload": "fasta-protein-%dMiB" % target_mib, "description": desc, "bytes": n,
+                   "sequences": info["sequences"], "residues": info["residues"], "blocks": info["blocks"],
+                   "lines": info["lines"]},
+        "stages_ms": {k: round(v, 3) for k, v in stages.items()},
+        "gpu_gbps": round(n / (stages["total_ms"] * 1e-3) / 1e9, 2),
+        "roofline": {"bound": "hbm", "kernel": "emit_kernel",
+                     "achieved": round(emit_bytes / (stages["emit_ms"] * 1e-3) / 1e9, 1), "peak": 8000.0,
+                     "unit": "GB/s", "frac": round(emit_bytes / (stages["emit_ms"] * 1e-3) / 8e12, 4),
+                     "traffic": measured_traffic("fasta-protein-%dMiB" % target_mib, "emit_kernel"),
+                     "bytes_per_launch": emit_bytes},
+        "parse_roofline": {"achieved": round((n + out_bytes) / (wall_ms * 1e-3) / 1e9, 1), "peak": 8000.0,
+                           "unit": "GB/s", "frac": round((n + out_bytes) / (wall_ms * 1e-3) / 8e12, 4)},
+        "bit_exact_vs_oracle": bool(same),
+    }
+    if cpu:
+        # A prefix of the text that ends at a record boundary.
+        cut = min(n, int(cpu_mib) << 20)
+        nxt = np.nonzero(text[cut:] == ord(">"))[0]
+        cut = cut + int(nxt[0]) if len(nxt) else n
+        sample = text[:cut].tobytes()
+        if M.ReferenceFasta.available():
+            with tempfile.NamedTemporaryFile(suffix=".fasta", delete=False) as f:
+                f.write(sample)
+                path = f.name
+            ref = M.ReferenceFasta()
+            ref.load(path)              # page cache warm
+            t0 = time.time()
+            reps = 0
+            while True:
+                ref.load(path)
+                reps += 1
+                if time.time() - t0 > 5.0:
+                    break
+            cpu_s = (time.time() - t0) / reps
+            os.unlink(path)
+            kind, what = "reference", "the reference's fasta.cpp + encoder.cpp + database.cpp (oracle/_ref)"
+        else:
+            ora = M.FastaOracle()
+            t0 = time.time()
+            ora.parse(sample)
+            cpu_s = time.time() - t0
+            kind, what = "port", "oracle/fasta_oracle.c"
+        gbps = len(sample) / cpu_s / 1e9
+        result["cpu_baseline"] = {"value": round(gbps, 4), "unit": "GB/s", "cores": 1, "kind": kind,
+                                  "sample": "the first %.1f MiB of the same text (%s), one thread" %
+                                            (len(sample) / 2**20, what)}
+        result["vs_cpu"] = round(result["value"] / gbps, 1)
+    ld.close()
+    d_text.close()
+    return result
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -315,6 +404,9 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=100.0, help="rate of the serving figure in the main line")
     ap.add_argument("--phylogeny", action="store_true",
                     help="only the neighbor-joining stage over the workload's scores (its own JSON line)")
+    ap.add_argument("--fasta", action="store_true",
+                    help="only the FASTA loader over a synthetic protein database (its own JSON line)")
+    ap.add_argument("--fasta-mib", type=int, default=512, help="size of the FASTA database for --fasta")
     ap.add_argument("--tuning", default=DEFAULT_TUNING,
                     help="tuning table for the batched plan (default: the committed gfx950 table; '' = built-in)")
     args = ap.parse_args()
@@ -338,6 +430,10 @@ def main():
                        "seconds_per_rate": args.serve_seconds, "slots": 4},
             "serving": sweep}), flush=True)
         return 0 if all(r["bit_exact_vs_alone"] for r in sweep) else 1
+    if args.fasta:
+        res = fasta_run(N, args.steps, args.warmup, args.fasta_mib, cpu=not args.no_cpu)
+        print(json.dumps(res), flush=True)
+        return 0 if res["bit_exact_vs_oracle"] else 1
     w = make_workload(args.workload, tables)
     if args.phylogeny:
         res = phylogeny_run(N, w, args.steps, args.warmup, cpu=not args.no_cpu)

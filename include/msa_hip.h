@@ -245,6 +245,53 @@ int msa_nj_build(const float* distances, int32_t n, int32_t variant, msa_nj_node
 int msa_nj_build_device(const int32_t* d_scores, int32_t n, int32_t variant, msa_nj_node* d_nodes,
                         msa_stream stream);
 
+/* ---- FASTA loading -------------------------------------------------------------
+ * Replaces, on the device, the reference's FASTA loader and its packing for the
+ * pairwise stage: io::parser::fasta (src/io/loader/parser/fasta.cpp:26-70),
+ * encoder::encode (src/encoder.cpp:50-98) and pairwise::database
+ * (src/pairwise/database.cu:24-72).  A loader parses FASTA text into the
+ * sequence store the msa_nw_* entry points take: every record's blocks back to
+ * back, offset and size per record in blocks, in file order, plus each
+ * record's description as a [start, end) byte range of the text.  Records and
+ * blocks are exactly those of the reference (lines end at '\n', '\r' is a
+ * residue, a record ends at the next '>' line or the first empty line, lines
+ * outside records are skipped).  A loader keeps its device buffers for the
+ * next parse; the results stay valid until then. */
+typedef struct msa_fasta_opaque* msa_fasta;
+
+typedef struct {
+    int64_t sequences;     /* records (database::count) */
+    int64_t blocks;        /* total blocks of the store */
+    int64_t residues;      /* total residues, without padding */
+    int64_t lines;
+    int64_t bytes;         /* text size */
+} msa_fasta_info;
+
+int msa_fasta_create(msa_fasta* loader);
+int msa_fasta_destroy(msa_fasta loader);
+/* Text already in device memory; returns when the store is complete. */
+int msa_fasta_parse_device(msa_fasta loader, const char* d_text, int64_t bytes, msa_stream stream);
+/* Host text: one copy to the device, then as above (null stream). */
+int msa_fasta_parse(msa_fasta loader, const char* text, int64_t bytes);
+int msa_fasta_info_get(msa_fasta loader, msa_fasta_info* info);
+/* GPU time of the last parse, from events on its stream: the whole parse, the
+ * line table (count + index), the record state, offsets and emit
+ * descriptors (with the host's readbacks in between), and the emit kernel that
+ * writes the blocks.  Zeros before the first non-empty parse. */
+typedef struct {
+    float total_ms;
+    float lines_ms;
+    float records_ms;
+    float emit_ms;
+} msa_fasta_timing;
+int msa_fasta_timing_get(msa_fasta loader, msa_fasta_timing* timing);
+/* Device pointers of the store (any may be NULL). */
+int msa_fasta_device_store(msa_fasta loader, const uint16_t** d_blocks, const int64_t** d_offset,
+                           const int32_t** d_size, const int64_t** d_desc_start, const int64_t** d_desc_end);
+/* Copies the store to host arrays sized from msa_fasta_info (any may be NULL). */
+int msa_fasta_copy(msa_fasta loader, uint16_t* blocks, int64_t* offset, int32_t* size, int64_t* desc_start,
+                   int64_t* desc_end);
+
 /* ---- serving -----------------------------------------------------------------
  * No counterpart in the reference (a batch MPI pipeline); its closest relative
  * is the free-memory admission of hybrid.cu:345-375.  A server keeps a device

@@ -72,6 +72,15 @@ class SrvJobStats(ctypes.Structure):
                 ("device_bytes", _i64), ("slot", _i32)]
 
 
+class FastaInfo(ctypes.Structure):
+    _fields_ = [("sequences", _i64), ("blocks", _i64), ("residues", _i64), ("lines", _i64), ("bytes", _i64)]
+
+
+class FastaTiming(ctypes.Structure):
+    _fields_ = [("total_ms", ctypes.c_float), ("lines_ms", ctypes.c_float), ("records_ms", ctypes.c_float),
+                ("emit_ms", ctypes.c_float)]
+
+
 class SrvStats(ctypes.Structure):
     _fields_ = [("submitted", _i64), ("completed", _i64), ("running", _i64), ("pending", _i64),
                 ("cells", _i64), ("device_bytes_used", _i64), ("device_bytes_capacity", _i64)]
@@ -117,6 +126,15 @@ SIGNATURES = {
     "msa_nw_tune": (ctypes.c_int, [_vp, _vp, _vp, _i32, _vp, _i32, _i64, ctypes.c_char_p, _vp, ctypes.c_size_t]),
     "msa_nj_build": (ctypes.c_int, [_vp, _i32, _i32, _vp]),
     "msa_nj_build_device": (ctypes.c_int, [_vp, _i32, _i32, _vp, _vp]),
+    "msa_fasta_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
+    "msa_fasta_destroy": (ctypes.c_int, [_vp]),
+    "msa_fasta_parse_device": (ctypes.c_int, [_vp, _vp, _i64, _vp]),
+    "msa_fasta_parse": (ctypes.c_int, [_vp, ctypes.c_char_p, _i64]),
+    "msa_fasta_info_get": (ctypes.c_int, [_vp, ctypes.POINTER(FastaInfo)]),
+    "msa_fasta_timing_get": (ctypes.c_int, [_vp, ctypes.POINTER(FastaTiming)]),
+    "msa_fasta_device_store": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                              ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+    "msa_fasta_copy": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "msa_srv_open": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(SrvConfig)]),
     "msa_srv_submit": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, ctypes.POINTER(ctypes.c_uint64)]),
     "msa_srv_poll": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]),
@@ -374,3 +392,61 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+class FastaLoader:
+    """msa_fasta_*: FASTA text parsed on the GPU into the sequence store the
+    msa_nw_* entry points take.  parse() takes host bytes, parse_device() a
+    device pointer; store() copies the result to numpy arrays."""
+
+    def __init__(self):
+        h = _vp()
+        check(lib().msa_fasta_create(ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().msa_fasta_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def parse(self, text):
+        raw = bytes(text)
+        check(lib().msa_fasta_parse(self.handle, raw, len(raw)))
+        return self.info()
+
+    def parse_device(self, d_text, nbytes, stream=None):
+        check(lib().msa_fasta_parse_device(self.handle, d_text, int(nbytes), stream))
+        return self.info()
+
+    def info(self):
+        inf = FastaInfo()
+        check(lib().msa_fasta_info_get(self.handle, ctypes.byref(inf)))
+        return {k: int(getattr(inf, k)) for k, _ in FastaInfo._fields_}
+
+    def timing(self):
+        t = FastaTiming()
+        check(lib().msa_fasta_timing_get(self.handle, ctypes.byref(t)))
+        return {k: float(getattr(t, k)) for k, _ in FastaTiming._fields_}
+
+    def device_store(self):
+        ptrs = [_vp() for _ in range(5)]
+        check(lib().msa_fasta_device_store(self.handle, *[ctypes.byref(p) for p in ptrs]))
+        return [p.value for p in ptrs]
+
+    def store(self):
+        """(blocks uint16, offset int64, size int32, desc_start int64, desc_end int64)."""
+        inf = self.info()
+        r = inf["sequences"]
+        blocks = np.zeros(max(inf["blocks"], 1), dtype=np.uint16)
+        offset = np.zeros(max(r, 1), dtype=np.int64)
+        size = np.zeros(max(r, 1), dtype=np.int32)
+        ds = np.zeros(max(r, 1), dtype=np.int64)
+        de = np.zeros(max(r, 1), dtype=np.int64)
+        check(lib().msa_fasta_copy(self.handle, _ptr(blocks), _ptr(offset), _ptr(size), _ptr(ds), _ptr(de)))
+        return blocks[:inf["blocks"]], offset[:r], size[:r], ds[:r], de[:r]

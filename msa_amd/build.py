@@ -22,7 +22,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MSA_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["runtime.hip", "nw_plain.hip", "nw_batched.hip", "nw_long.hip", "nw_plan.hip", "nw_serve.hip",
-               "nw_tune.hip", "nj.hip"]
+               "nw_tune.hip", "nj.hip", "fasta.hip"]
 HIP_FLAGS = ["-std=c++17", "-O3", "-fPIC", "--offload-arch=" + ARCH,
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-command-line-argument"]

@@ -205,3 +205,44 @@ def poisson_arrivals(rate, count, seed=50):
     """Arrival times (seconds) of a Poisson stream of the given rate."""
     rng = SplitMix64(seed)
     return np.cumsum(-np.log1p(-rng.uniform(count)) / rate)
+
+
+def fasta_block(rng, target_bytes, width=60, median=400.0, sigma=1.0, lo=50, hi=5000):
+    """FASTA text of about `target_bytes`: protein records with lengths
+    clip(round(lognormal(ln median, sigma)), lo, hi), BLOSUM62 background
+    residues as upper-case letters, `width` residues per line, headers
+    '>s' + 12-digit record number.  Built with array arithmetic (no per-record
+    Python loop); returns a uint8 array."""
+    hdr = 15
+    est = int(target_bytes // (hdr + median)) + 64      # more records than needed; cut below
+    lengths = lognormal_lengths(rng, est, median, sigma, lo, hi)
+    per = hdr + lengths + (lengths + width - 1) // width
+    cum = np.cumsum(per)
+    count = max(1, int(np.searchsorted(cum, target_bytes, side="right")))
+    lengths, per, cum = lengths[:count], per[:count], cum[:count]
+    total = int(cum[-1])
+    base = cum - per
+    out = np.full(total, ord("\n"), dtype=np.uint8)
+    out[base] = ord(">")
+    out[base + 1] = ord("s")
+    ids = np.arange(count, dtype=np.int64)
+    for d in range(12):
+        out[base + 2 + d] = (ids // 10 ** (11 - d)) % 10 + ord("0")
+    n = int(lengths.sum())
+    letters = np.frombuffer(AMINO20.encode(), dtype=np.uint8)[rng.categorical(BLOSUM62_BACKGROUND, n)]
+    rid = np.repeat(ids, lengths)
+    k = np.arange(n, dtype=np.int64) - np.repeat(np.cumsum(lengths) - lengths, lengths)
+    out[base[rid] + hdr + k + k // width] = letters
+    return out
+
+
+def fasta_database(seed=6, target_bytes=512 << 20, block_bytes=64 << 20, width=60):
+    """A protein FASTA database of about `target_bytes`: one `fasta_block` of
+    `block_bytes` (seed `seed`) repeated.  Returns (uint8 text, description)."""
+    rng = SplitMix64(seed)
+    block = fasta_block(rng, min(block_bytes, target_bytes), width=width)
+    reps = max(1, int(round(target_bytes / len(block))))
+    text = np.tile(block, reps)
+    desc = ("protein FASTA, %d x a %.1f MiB block of records with L=clip(round(lognormal(ln400,1)),50,5000), "
+            "BLOSUM62 background, %d residues per line, seed %d" % (reps, len(block) / 2**20, width, seed))
+    return text, desc

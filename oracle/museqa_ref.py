@@ -24,6 +24,13 @@ Contents and what each follows:
   tree_splits / additive_tree ........... textbook helpers that pin NjOracle:
                                           the bipartitions of a tree and the
                                           path-length metric of a random tree
+  FastaOracle ........................... ctypes wrapper of fasta_oracle.c, the
+                                          C restatement of fasta.cpp:26-70 +
+                                          encoder.cpp:76-98 + the database
+                                          packing, over bytes in memory
+  ReferenceFasta ........................ ctypes wrapper of oracle/_ref/
+                                          libref_fasta.so: the reference's own
+                                          fasta.cpp, encoder.cpp, database.cpp
 """
 import ctypes
 import json
@@ -368,3 +375,59 @@ def additive_tree(rng, n, lo=1, hi=20):
     packed = np.array([dist[i, j] for i in range(1, n) for j in range(i)], dtype=np.float32)
     splits = {s for s in side_len if 1 < len(s) < n - 1}
     return packed, splits, side_len
+
+
+class FastaOracle:
+    """ctypes wrapper of fasta_oracle.c (in oracle/_build/liboracle.so)."""
+
+    def __init__(self, path=None):
+        path = path or os.path.join(HERE, "_build", "liboracle.so")
+        self.lib = ctypes.CDLL(path)
+        self.lib.fasta_oracle.restype = ctypes.c_int64
+        self.lib.fasta_oracle.argtypes = [ctypes.c_void_p, ctypes.c_int64] + [ctypes.c_void_p] * 6
+
+    def parse(self, text):
+        """(blocks uint16, offset int64, size int32, desc_start int64, desc_end int64)
+        of the text (bytes), in the layout of the pairwise sequence store."""
+        t = np.frombuffer(bytes(text), dtype=np.uint8) if len(text) else np.zeros(1, np.uint8)
+        nb = ctypes.c_int64()
+        r = self.lib.fasta_oracle(t.ctypes.data, len(text), None, None, None, None, None, ctypes.byref(nb))
+        blocks = np.zeros(max(nb.value, 1), np.uint16)
+        arrays = [np.zeros(max(r, 1), dt) for dt in (np.int64, np.int32, np.int64, np.int64)]
+        self.lib.fasta_oracle(t.ctypes.data, len(text), blocks.ctypes.data, *[a.ctypes.data for a in arrays],
+                              ctypes.byref(nb))
+        return (blocks[:nb.value],) + tuple(a[:r] for a in arrays)
+
+
+class ReferenceFasta:
+    """ctypes wrapper of oracle/_ref/libref_fasta.so (oracle/Makefile, target
+    `ref`): the reference's own FASTA parser over a file.  Present only where
+    it was built."""
+
+    PATH = os.path.join(HERE, "_ref", "libref_fasta.so")
+
+    @classmethod
+    def available(cls):
+        return os.path.exists(cls.PATH)
+
+    def __init__(self):
+        self.lib = ctypes.CDLL(self.PATH)
+        self.lib.ref_fasta_load.restype = ctypes.c_int64
+        self.lib.ref_fasta_load.argtypes = [ctypes.c_char_p] + [ctypes.c_void_p] * 7
+
+    def load(self, path):
+        """(blocks, offset, size, descriptions) of the file, or None on error."""
+        nb, db = ctypes.c_int64(), ctypes.c_int64()
+        name = os.fsencode(path)
+        r = self.lib.ref_fasta_load(name, None, None, None, None, None, ctypes.byref(nb), ctypes.byref(db))
+        if r < 0:
+            return None
+        blocks = np.zeros(max(nb.value, 1), np.uint16)
+        offset = np.zeros(max(r, 1), np.int64)
+        size = np.zeros(max(r, 1), np.int32)
+        desc = ctypes.create_string_buffer(max(db.value, 1))
+        doff = np.zeros(r + 1, np.int64)
+        self.lib.ref_fasta_load(name, blocks.ctypes.data, offset.ctypes.data, size.ctypes.data, desc,
+                                doff.ctypes.data, ctypes.byref(nb), ctypes.byref(db))
+        descs = [desc.raw[doff[i]:doff[i + 1]] for i in range(r)]
+        return blocks[:nb.value], offset[:r], size[:r], descs
