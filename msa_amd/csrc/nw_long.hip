@@ -34,7 +34,6 @@ namespace msa {
 namespace {
 
 constexpr int kLanes = 64;
-constexpr int kMaxSlots = 26;
 constexpr int kSpinLimit = 1 << 24;     // ~1e9 cycles of waiting before giving up
 
 struct LongArgs {
