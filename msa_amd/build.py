@@ -52,7 +52,7 @@ def build_hip(jobs=4):
             raise RuntimeError("hipcc failed")
     out = os.path.join(PKG, "libmsa_hip.so")
     tmp = out + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs)
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + ["-pthread"])
     os.replace(tmp, out)
     return out
 

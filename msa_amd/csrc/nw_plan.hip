@@ -23,8 +23,10 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <new>
+#include <thread>
 #include <utility>
 
 namespace msa {
@@ -32,6 +34,41 @@ namespace msa {
 namespace {
 
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+// Host threads for planning `work` pair visits: one below 2^20 (serving-size
+// jobs plan on their submitting thread), else up to 16.
+int plan_threads(int64_t work)
+{
+    if (work < (int64_t(1) << 20)) return 1;
+    const unsigned hw = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(hw ? hw : 1u, 16u));
+}
+
+// Runs f(t) for t = 0..T-1, thread 0 on the caller; false if any call threw
+// (an allocation failure), after all have finished.
+template <typename F>
+bool run_parallel(int T, F&& f)
+{
+    std::atomic<bool> ok{true};
+    auto guarded = [&](int t) {
+        try {
+            f(t);
+        } catch (...) {
+            ok = false;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < T; ++t) {
+        try {
+            pool.emplace_back(guarded, t);
+        } catch (...) {
+            guarded(t);      // no thread available: run it here
+        }
+    }
+    guarded(0);
+    for (auto& th : pool) th.join();
+    return ok;
+}
 
 }  // namespace
 
@@ -142,13 +179,17 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
     MSA_TRY(make_sxx(table625, penalty, hp.sxx, symmetric));
     hp.penalty = penalty;
 
-    // Classify pairs (host only).
+    // Classify pairs (host only), into parts: one for an explicit list, one
+    // per planning thread (a contiguous range of queries) for all pairs.
     struct Fast { int32_t query, target, index; };
-    std::vector<Fast> fast, longs;
-    fast.reserve(kernel == MSA_KERNEL_PLAIN ? 0 : (size_t)npairs);
+    struct Part {
+        std::vector<Fast> fast, longs;
+        std::vector<int32_t> plain_pairs, plain_index;
+        int64_t cells = 0;
+    };
+    std::vector<Part> parts;
     const int64_t long_threshold = kernel == MSA_KERNEL_LONG ? 0
                                  : (opt.long_threshold > 0 ? opt.long_threshold : tuning->long_threshold);
-    int64_t cells = 0;
     // Targets of one query are queued longest first: sequences ranked by
     // (rows descending, id descending).
     std::vector<int32_t> by_len(nseq), rank(nseq);
@@ -175,21 +216,25 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
         const bool is_long = kernel != MSA_KERNEL_BATCHED && std::min(r.first_end, col.last_real) >= long_threshold;
         return {is_long ? kLong : kFast, q * 2 + transpose, t, c};
     };
-    auto add_plain = [&](int32_t first, int32_t second, int64_t idx) {
-        hp.plain_pairs.push_back(first);
-        hp.plain_pairs.push_back(second);
-        hp.plain_index.push_back((int32_t)idx);
+    auto add_plain = [](Part& P, int32_t first, int32_t second, int64_t idx) {
+        P.plain_pairs.push_back(first);
+        P.plain_pairs.push_back(second);
+        P.plain_index.push_back((int32_t)idx);
     };
     if (pairs) {
+        parts.resize(1);
+        Part& P = parts[0];
+        std::vector<Fast>& fast = P.fast;
+        if (kernel != MSA_KERNEL_PLAIN) fast.reserve((size_t)npairs);
         for (int64_t c = 0; c < npairs; ++c) {
             const int32_t first = pairs[2 * c], second = pairs[2 * c + 1];
             if (first < 0 || first >= nseq || second < 0 || second >= nseq)
                 return fail(MSA_ERR_INVALID_ARGUMENT, "pair %lld names sequence outside 0..%d", (long long)c,
                             nseq - 1);
             const Route rt = route(first, second);
-            cells += rt.cells;
-            if (rt.kind == kPlain) add_plain(first, second, c);
-            else (rt.kind == kLong ? longs : fast).push_back({rt.qkey, rt.target, (int32_t)c});
+            P.cells += rt.cells;
+            if (rt.kind == kPlain) add_plain(P, first, second, c);
+            else (rt.kind == kLong ? P.longs : fast).push_back({rt.qkey, rt.target, (int32_t)c});
         }
         // Group the batched pairs by query key, targets in rank order, list
         // order within a target: two stable counting passes.
@@ -209,34 +254,58 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
         // rank order, so the batched list comes out grouped and ordered with
         // no sort.  Every pair is met from both of its sequences: batched and
         // long pairs are taken on their query's side, plain pairs on the side
-        // of their larger index.
-        std::vector<Fast> transposed;
-        for (int32_t q = 0; q < nseq; ++q) {
-            transposed.clear();
-            for (int32_t k = 0; k < nseq; ++k) {
-                const int32_t t = by_len[k];
-                if (t == q) continue;
-                const int32_t first = std::max(q, t), second = std::min(q, t);
-                const int64_t idx = (int64_t)first * (first - 1) / 2 + second;
-                const Route rt = route(first, second);
-                if (rt.kind == kPlain) {
-                    if (q == first) cells += rt.cells, add_plain(first, second, idx);
-                    continue;
+        // of their larger index.  Threads take contiguous query ranges (every
+        // query visits all nseq partners, so ranges of equal size balance).
+        const int T = plan_threads((int64_t)nseq * nseq);
+        parts.resize(T);
+        const bool ok = run_parallel(T, [&](int t) {
+            Part& P = parts[t];
+            const int32_t q0 = (int32_t)((int64_t)nseq * t / T), q1 = (int32_t)((int64_t)nseq * (t + 1) / T);
+            if (kernel != MSA_KERNEL_PLAIN) P.fast.reserve((size_t)npairs / T + 16);
+            std::vector<Fast> transposed;
+            for (int32_t q = q0; q < q1; ++q) {
+                transposed.clear();
+                for (int32_t k = 0; k < nseq; ++k) {
+                    const int32_t tgt = by_len[k];
+                    if (tgt == q) continue;
+                    const int32_t first = std::max(q, tgt), second = std::min(q, tgt);
+                    const int64_t idx = (int64_t)first * (first - 1) / 2 + second;
+                    const Route rt = route(first, second);
+                    if (rt.kind == kPlain) {
+                        if (q == first) P.cells += rt.cells, add_plain(P, first, second, idx);
+                        continue;
+                    }
+                    if (rt.qkey >> 1 != q) continue;
+                    P.cells += rt.cells;
+                    const Fast f{rt.qkey, rt.target, (int32_t)idx};
+                    if (rt.kind == kLong) P.longs.push_back(f);
+                    else if (rt.qkey & 1) transposed.push_back(f);
+                    else P.fast.push_back(f);
                 }
-                if (rt.qkey >> 1 != q) continue;
-                cells += rt.cells;
-                const Fast f{rt.qkey, rt.target, (int32_t)idx};
-                if (rt.kind == kLong) longs.push_back(f);
-                else if (rt.qkey & 1) transposed.push_back(f);
-                else fast.push_back(f);
+                P.fast.insert(P.fast.end(), transposed.begin(), transposed.end());
             }
-            fast.insert(fast.end(), transposed.begin(), transposed.end());
-        }
+        });
+        if (!ok) return fail(MSA_ERR_OUT_OF_MEMORY, "host allocation failed while planning");
     }
+    // Plain and long pairs of every part, in part order.
+    std::vector<Fast> longs;
+    int64_t cells = 0, nfast = 0;
+    std::vector<int64_t> fast_at(parts.size() + 1, 0);     // offset of each part in the target list
+    for (size_t t = 0; t < parts.size(); ++t) {
+        Part& P = parts[t];
+        cells += P.cells;
+        hp.plain_pairs.insert(hp.plain_pairs.end(), P.plain_pairs.begin(), P.plain_pairs.end());
+        hp.plain_index.insert(hp.plain_index.end(), P.plain_index.begin(), P.plain_index.end());
+        longs.insert(longs.end(), P.longs.begin(), P.longs.end());
+        std::vector<int32_t>().swap(P.plain_pairs);
+        std::vector<int32_t>().swap(P.plain_index);
+        fast_at[t + 1] = fast_at[t] + (int64_t)P.fast.size();
+    }
+    nfast = fast_at.back();
     hp.info.pairs = npairs;
     hp.info.cells = cells;
     hp.info.plain_pairs = (int64_t)hp.plain_index.size();
-    hp.info.batched_pairs = (int64_t)fast.size();
+    hp.info.batched_pairs = nfast;
     hp.info.long_pairs = (int64_t)longs.size();
 
     DeviceShape dev;
@@ -262,7 +331,7 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
     // Batched part: group by (query, transpose), sort each query's targets by
     // length, cut into items of up to 16 targets, queue the items costliest
     // first.  Each item carries the tile shape of its query's length bin.
-    if (!fast.empty()) {
+    if (nfast > 0) {
         // Targets per item: 16 (four rounds of four) when the plan has enough
         // pairs to give every resident wave four items, else 8 or 4, so that
         // small plans (serving jobs) still spread over the whole GPU.
@@ -271,42 +340,64 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
         if (per_cu < 1) return fail(MSA_ERR_RUNTIME, "batched kernel does not fit a compute unit");
         const int64_t waves = (int64_t)per_cu * dev.compute_units * (batched_threads() / 64);
         int item_targets = 16;
-        while (item_targets > 4 && (int64_t)fast.size() < (int64_t)item_targets * 4 * waves) item_targets /= 2;
+        while (item_targets > 4 && nfast < (int64_t)item_targets * 4 * waves) item_targets /= 2;
         struct Item { int4 v; int64_t cost; };
+        struct PartItems {
+            std::vector<Item> items;
+            int32_t max_t = 1;
+            bool need_lines = false;
+            unsigned used = 0;
+        };
+        std::vector<PartItems> pitems(parts.size());
+        hp.targets.resize((size_t)nfast);
+        // Every part holds whole queries, so parts build their items alone;
+        // an item's start is its offset in the whole target list.
+        const bool ok = run_parallel((int)parts.size(), [&](int t) {
+            const std::vector<Fast>& fast = parts[t].fast;
+            PartItems& out = pitems[t];
+            const int64_t base = fast_at[t];
+            for (size_t k = 0; k < fast.size(); ++k) {
+                hp.targets[base + k] = make_int2(fast[k].target, fast[k].index);
+                out.max_t = std::max(out.max_t, seqs[fast[k].target].first_end);
+            }
+            size_t k = 0;
+            while (k < fast.size()) {
+                size_t e = k;
+                while (e < fast.size() && fast[e].query == fast[k].query) ++e;
+                const int32_t q = fast[k].query >> 1;
+                const int32_t lq = seqs[q].first_end;
+                const int cfg = forced >= 0 ? forced : tuning_config_for(*tuning, lq);
+                const TileConfig tc = kTileConfig[cfg];
+                const int groups = 64 / tc.lanes;
+                out.used |= 1u << cfg;
+                out.need_lines = out.need_lines || lq > tc.lanes * tc.rows;
+                const int32_t flags = (fast[k].query & 1) | (cfg << 1);
+                // Rows computed: the query rounded up to 4 rows per lane.
+                const int64_t rows = ((int64_t)lq + 4 * tc.lanes - 1) / (4 * tc.lanes) * (4 * tc.lanes);
+                for (size_t s = k; s < e; s += item_targets) {
+                    const int32_t cnt = (int32_t)std::min<size_t>(item_targets, e - s);
+                    int64_t cost = 0;
+                    // A round of `groups` targets costs as much as its longest (the first).
+                    for (size_t u = s; u < s + cnt; u += groups) {
+                        const int64_t chunks = (seqs[fast[u].target].first_end + 2 * tc.lanes - 2) / tc.lanes;
+                        cost += rows * chunks * tc.lanes;
+                    }
+                    out.items.push_back({make_int4(q, (int)(base + s), cnt, flags), cost});
+                }
+                k = e;
+            }
+            std::vector<Fast>().swap(parts[t].fast);
+        });
+        if (!ok) return fail(MSA_ERR_OUT_OF_MEMORY, "host allocation failed while planning");
         std::vector<Item> items;
-        hp.targets.resize(fast.size());
         int32_t max_t = 1;
-        for (size_t k = 0; k < fast.size(); ++k) {
-            hp.targets[k] = make_int2(fast[k].target, fast[k].index);
-            max_t = std::max(max_t, seqs[fast[k].target].first_end);
-        }
         bool need_lines = false;
         unsigned used = 0;
-        size_t k = 0;
-        while (k < fast.size()) {
-            size_t e = k;
-            while (e < fast.size() && fast[e].query == fast[k].query) ++e;
-            const int32_t q = fast[k].query >> 1;
-            const int32_t lq = seqs[q].first_end;
-            const int cfg = forced >= 0 ? forced : tuning_config_for(*tuning, lq);
-            const TileConfig tc = kTileConfig[cfg];
-            const int groups = 64 / tc.lanes;
-            used |= 1u << cfg;
-            need_lines = need_lines || lq > tc.lanes * tc.rows;
-            const int32_t flags = (fast[k].query & 1) | (cfg << 1);
-            // Rows computed: the query rounded up to 4 rows per lane.
-            const int64_t rows = ((int64_t)lq + 4 * tc.lanes - 1) / (4 * tc.lanes) * (4 * tc.lanes);
-            for (size_t s = k; s < e; s += item_targets) {
-                const int32_t cnt = (int32_t)std::min<size_t>(item_targets, e - s);
-                int64_t cost = 0;
-                // A round of `groups` targets costs as much as its longest (the first).
-                for (size_t u = s; u < s + cnt; u += groups) {
-                    const int64_t chunks = (seqs[fast[u].target].first_end + 2 * tc.lanes - 2) / tc.lanes;
-                    cost += rows * chunks * tc.lanes;
-                }
-                items.push_back({make_int4(q, (int)s, cnt, flags), cost});
-            }
-            k = e;
+        for (const PartItems& pi : pitems) {
+            items.insert(items.end(), pi.items.begin(), pi.items.end());
+            max_t = std::max(max_t, pi.max_t);
+            need_lines = need_lines || pi.need_lines;
+            used |= pi.used;
         }
         std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.cost > y.cost; });
         hp.items.resize(items.size());
