@@ -327,10 +327,10 @@ namespace museqa
             template <int K>
             std::unique_ptr<algorithm> make_hip() { return std::unique_ptr<algorithm>(new hip_algorithm(K)); }
 
-            // src/pairwise/pairwise.cpp:26-35 plus the GPU variants.  The
-            // reference's sequential/distributed names run the plain kernel:
-            // one pair per workgroup, every padding rule of sequential.cpp,
-            // identical scores.
+            // src/pairwise/pairwise.cpp:26-35 plus the GPU variants.  Every
+            // route gives the scores of sequential.cpp exactly, so the
+            // reference's names all take the auto-routed path (the fastest);
+            // hip-plain / hip-batched / hip-long force one kernel family.
             const std::vector<std::pair<std::string, factory>>& registry()
             {
                 static const std::vector<std::pair<std::string, factory>> r = {
@@ -338,10 +338,10 @@ namespace museqa
                     {"needleman", make_hip<MSA_KERNEL_AUTO>},
                     {"hybrid", make_hip<MSA_KERNEL_AUTO>},
                     {"needleman-hybrid", make_hip<MSA_KERNEL_AUTO>},
-                    {"sequential", make_hip<MSA_KERNEL_PLAIN>},
-                    {"distributed", make_hip<MSA_KERNEL_PLAIN>},
-                    {"needleman-sequential", make_hip<MSA_KERNEL_PLAIN>},
-                    {"needleman-distributed", make_hip<MSA_KERNEL_PLAIN>},
+                    {"sequential", make_hip<MSA_KERNEL_AUTO>},
+                    {"distributed", make_hip<MSA_KERNEL_AUTO>},
+                    {"needleman-sequential", make_hip<MSA_KERNEL_AUTO>},
+                    {"needleman-distributed", make_hip<MSA_KERNEL_AUTO>},
                     {"hip", make_hip<MSA_KERNEL_AUTO>},
                     {"hip-plain", make_hip<MSA_KERNEL_PLAIN>},
                     {"hip-batched", make_hip<MSA_KERNEL_BATCHED>},
