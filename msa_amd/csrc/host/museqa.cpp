@@ -201,6 +201,52 @@ namespace museqa
         return result;
     }
 
+    database database::load_on_device(const std::string& filename)
+    {
+        const auto dot = filename.find_last_of('.');
+        const std::string ext = filename.empty() ? std::string {}
+                              : filename.substr(dot == std::string::npos ? 0 : dot + 1);
+        if (ext != "fa" && ext != "fasta") raise("unknown database parser '%s'", ext);
+        std::ifstream file(filename, std::ios::binary | std::ios::ate);
+        if (!file) raise("file does not exist or cannot be read '%s'", filename);
+        std::string text(size_t(file.tellg()), '\0');
+        file.seekg(0);
+        if (!file.read(&text[0], std::streamsize(text.size()))) raise("cannot read '%s'", filename);
+        return io::parser::fasta_on_device(text);
+    }
+
+    // The GPU loader builds the sequence store; records become entries in
+    // file order, with database::add's key rule.
+    database io::parser::fasta_on_device(const std::string& text)
+    {
+        int devices = 0;
+        check(msa_device_count(&devices));
+        if (devices < 1) throw exception("no GPU available for the FASTA loader");
+        msa_fasta loader = nullptr;
+        check(msa_fasta_create(&loader));
+        struct closer {
+            msa_fasta f;
+            ~closer() { msa_fasta_destroy(f); }
+        } guard {loader};
+        check(msa_fasta_parse(loader, text.data(), int64_t(text.size())));
+        msa_fasta_info info {};
+        check(msa_fasta_info_get(loader, &info));
+        const size_t n = size_t(info.sequences);
+        std::vector<uint16_t> blocks(size_t(info.blocks));
+        std::vector<int64_t> offset(n), start(n), end(n);
+        std::vector<int32_t> size(n);
+        check(msa_fasta_copy(loader, blocks.data(), offset.data(), size.data(), start.data(), end.data()));
+        database result;
+        result.m_entries.reserve(n);
+        for (size_t r = 0; r < n; ++r) {
+            encoder::buffer b(blocks.begin() + offset[r], blocks.begin() + offset[r] + size[r]);
+            std::string description = text.substr(size_t(start[r]), size_t(end[r] - start[r]));
+            result.m_indeces[description] = ptrdiff_t(r);
+            result.m_entries.push_back({std::move(description), sequence {std::move(b)}});
+        }
+        return result;
+    }
+
     namespace pairwise
     {
         // ---- distance matrix (src/pairwise/pairwise.cuh:53-117) -----------

@@ -64,6 +64,7 @@ namespace museqa
 
             sequence() = default;
             sequence(const encoder::buffer& blocks) : m_blocks {blocks} {}
+            sequence(encoder::buffer&& blocks) noexcept : m_blocks {std::move(blocks)} {}
             sequence(const char* ptr, size_t size) : m_blocks {encoder::encode(ptr, size)} {}
             sequence(const std::string& str) : sequence {str.data(), str.size()} {}
 
@@ -79,6 +80,9 @@ namespace museqa
         private:
             encoder::buffer m_blocks;
     };
+
+    class database;
+    namespace io { namespace parser { database fasta_on_device(const std::string& text); } }
 
     class database
     {
@@ -105,8 +109,12 @@ namespace museqa
 
             // io::loader<database> for ".fa" / ".fasta" files (src/io/loader/database.cpp).
             static database load(const std::string& filename);
+            // The same database, parsed on the GPU (msa_fasta_*): identical
+            // records and contents; ".fa" / ".fasta" files only.
+            static database load_on_device(const std::string& filename);
 
         private:
+            friend database io::parser::fasta_on_device(const std::string&);
             static int anonymous;
             std::vector<entry_type> m_entries;
             std::map<std::string, ptrdiff_t> m_indeces;
@@ -117,6 +125,8 @@ namespace museqa
         namespace parser
         {
             database fasta(const std::string& filename);
+            // FASTA text already in memory, parsed on the GPU.
+            database fasta_on_device(const std::string& text);
         }
     }
 

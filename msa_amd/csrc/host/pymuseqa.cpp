@@ -46,6 +46,10 @@ PYBIND11_MODULE(_museqa, m)
             return d.only(std::set<std::string>(k.begin(), k.end()));
         })
         .def_static("load", &database::load)
+        .def_static("load_on_device", [](const std::string& name) {
+            py::gil_scoped_release release;
+            return database::load_on_device(name);
+        })
         .def_property_readonly("count", &database::count);
 
     py::class_<pairwise::scoring_table>(m, "ScoringTable")

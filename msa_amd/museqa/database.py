@@ -49,10 +49,15 @@ class Database:
         return Database(self._db.only_keys([k.decode() if isinstance(k, bytes) else k for k in keys]))
 
     @staticmethod
-    def load(*filenames):
+    def load(*filenames, device=False):
+        """Loads FASTA files (src/io/loader/parser/fasta.cpp rules); with
+        device=True the text is parsed on the GPU into the same database."""
+        loader = core.Database.load_on_device if device else core.Database.load
+        if len(filenames) == 1:
+            return Database(loader(filenames[0]))
         result = core.Database()
         for name in filenames:
-            result.merge(core.Database.load(name))
+            result.merge(loader(name))
         return Database(result)
 
     @property

@@ -121,3 +121,30 @@ def test_store_feeds_the_pairwise_stage(loader):
         t, p = N.table("blosum62")
         scores = N.align(blocks, offset, size, t, p)
         assert [int(v) for v in scores] == golden["%s/blosum62" % fixture]["scores"], fixture
+
+
+def _entries(db):
+    return [(e.description, str(e.contents)) for e in (db[i] for i in range(db.count))]
+
+
+def test_database_load_on_device_matches_the_host_loader(tmp_path):
+    """museqa::database::load_on_device builds the database the host loader
+    (the reference's rules) builds: same records, descriptions and contents,
+    same key rule for repeated descriptions."""
+    from msa_amd.museqa.database import Database
+    for fixture in ("mock1.fasta", "mock2.fasta", "mock3.fasta"):
+        path = os.path.join(H.GOLDEN, fixture)
+        assert _entries(Database.load(path, device=True)) == _entries(Database.load(path))
+    with open(os.path.join(H.GOLDEN, "loader_md5.json")) as f:
+        digests = json.load(f)["digests"]
+    for fixture, expected in digests.items():
+        db = Database.load(os.path.join(H.GOLDEN, fixture), device=True)
+        for key, digest in expected.items():
+            assert digest == hashlib.md5(str(db[key].contents).encode()).hexdigest()
+    rng = np.random.default_rng(21)
+    text = H.synthetic_fasta(rng, 300, 0, 2500, width=70, crlf=True, blank_every=9, junk=True)
+    text += b">seq5 repeated description\nACGT\n"
+    path = tmp_path / "synthetic.fasta"
+    path.write_bytes(text)
+    host, dev = Database.load(str(path)), Database.load(str(path), device=True)
+    assert host.count == dev.count and _entries(host) == _entries(dev)

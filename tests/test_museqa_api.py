@@ -138,6 +138,11 @@ def testIfRaisesOnUnknownFile():
         Database.load("unknown/file/path")
     with pytest.raises(RuntimeError):
         Database.load("missing-file.fasta")
+    # The GPU loader checks the name and the file before it touches a device.
+    with pytest.raises(RuntimeError, match="unknown database parser"):
+        Database.load("unknown/file/path", device=True)
+    with pytest.raises(RuntimeError, match="does not exist"):
+        Database.load("missing-file.fasta", device=True)
 
 
 # ---- pairwise surface (no kernel launch) -------------------------------------
