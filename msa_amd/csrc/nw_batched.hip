@@ -27,8 +27,8 @@
 // Tile configurations.  Each item carries its tile shape, picked per query
 // length by the planner from the tuning table: G lanes per pair (16 or 32) and
 // at most RMAX rows per lane (16 or 32), so a full strip is G*RMAX rows.  The
-// first strip of a query takes R0 rows per lane, R0 a multiple of 4, so at
-// most 4*G - 1 padding rows are computed per query.
+// first strip of a query takes R0 rows per lane, R0 a multiple of kRowStep (2),
+// so at most 2*G - 1 padding rows are computed per query.
 //
 // Neutral padding.  The recurrence runs in the shifted domain G = H +
 // (i+j)*penalty, where every boundary cell is 0 and every cell is
@@ -53,7 +53,6 @@ constexpr int kCodes = 25;           // codes 0..23 and the neutral code 24
 constexpr int kNull = 24;
 constexpr int kWaves = 4;            // waves per workgroup (independent)
 constexpr int kMaxGroups = 4;        // lane groups per wave at 16 lanes
-constexpr int kRStep = 4;            // rows-per-lane granularity of first strips
 constexpr int kProfWords = kCodes * kMaxTileBytes / 4;   // one wave's profile region
 
 static_assert(kTileConfigCount == 3, "tile table and kernel dispatch must agree");
@@ -90,13 +89,15 @@ __device__ __forceinline__ int row_bcast(int v)
     return __builtin_amdgcn_mov_dpp(v, dev::kRowNewBcast0 + U, 0xF, 0xF, false);
 }
 
-// Profile layout inside one code's block of G*R bytes: the lanes' 16-byte
-// planes first ([plane][lane][4 words]), then an 8-byte plane ([lane][2
-// words]) when R % 16 >= 8, then a 4-byte plane ([lane][word]) when R % 8 == 4.
+// Profile layout inside one code's block of G*W words (W = ceil(R/4) words of
+// four rows per lane): the lanes' 16-byte planes first ([plane][lane][4
+// words]), then an 8-byte plane ([lane][2 words]) when W % 4 >= 2, then a
+// 4-byte plane ([lane][word]) when W is odd.
 template <int G, int R>
-__device__ __forceinline__ void load_prof(uint32_t (&w)[R / 4], const dev::lds_u32* code_base, int k)
+__device__ __forceinline__ void load_prof(uint32_t (&w)[dev::prof_words<R>], const dev::lds_u32* code_base, int k)
 {
-    constexpr int F = R / 16, H = (R % 16) / 8, Q = (R % 8) / 4;
+    constexpr int W = dev::prof_words<R>;
+    constexpr int F = W / 4, H = (W % 4) / 2, Q = W % 2;
 #pragma unroll
     for (int j = 0; j < F; ++j) {
         const dev::u32x4 v = *(const dev::lds_u32x4*)(code_base + j * G * 4 + k * 4);
@@ -117,19 +118,20 @@ __device__ __forceinline__ void load_prof(uint32_t (&w)[R / 4], const dev::lds_u
 template <int G, int R>
 __device__ __forceinline__ int prof_word(int k, int w)
 {
-    constexpr int F = R / 16, H = (R % 16) / 8;
+    constexpr int W = dev::prof_words<R>;
+    constexpr int F = W / 4, H = (W % 4) / 2;
     if (w < 4 * F) return (w >> 2) * G * 4 + k * 4 + (w & 3);
     if (H && w < 4 * F + 2) return F * G * 4 + k * 2 + (w - 4 * F);
     return F * G * 4 + H * G * 2 + k;
 }
 
 // Builds the profile of one strip: rows [row0, row0 + G*R) of the padded
-// query (rows below pad_top are neutral).
+// query (rows below pad_top are neutral; bytes past a lane's R rows are 0).
 template <int G, int R>
 __device__ __forceinline__ void build_profile(dev::lds_u32* prof, const dev::lds_u8* tab,
                               const uint16_t* __restrict__ qseq, int row0, int pad_top, bool transpose, int lane)
 {
-    constexpr int W = R / 4;
+    constexpr int W = dev::prof_words<R>;
     constexpr int CODE_WORDS = G * W;
     for (int slot = lane; slot < G * W; slot += 64) {
         const int k = slot / W;
@@ -138,7 +140,7 @@ __device__ __forceinline__ void build_profile(dev::lds_u32* prof, const dev::lds
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const int q = row0 + k * R + 4 * w + t - pad_top;
-            const int code = q >= 0 ? dev::code_at(qseq, q) : -1;
+            const int code = (q >= 0 && 4 * w + t < R) ? dev::code_at(qseq, q) : -1;
             codes[t] = code < kNull ? code : -1;
         }
         dev::lds_u32* dst = prof + prof_word<G, R>(k, w);
@@ -195,7 +197,7 @@ __device__ __forceinline__ int run_strip(const dev::lds_u32* prof, const TargetS
                          int wchunks, int k)
 {
     static_assert(G == 16 || G == 32, "lane groups of 16 or 32");
-    constexpr int W = R / 4;
+    constexpr int W = dev::prof_words<R>;
     constexpr int CODE_WORDS = G * W;
     int h[R];
 #pragma unroll
@@ -273,8 +275,8 @@ __device__ __forceinline__ int run_strip(const dev::lds_u32* prof, const TargetS
     return result;
 }
 
-// First strips: R0 rows per lane, R0 in kRStep..RMAX.
-template <int G, int RMAX, bool STORE, int R = kRStep>
+// First strips: R0 rows per lane, R0 in kRowStep..RMAX.
+template <int G, int RMAX, bool STORE, int R = kRowStep>
 __device__ __forceinline__ int run_first_strip(int r0, const dev::lds_u32* prof, const TargetState& t, int32_t* line, int wchunks,
                                int k)
 {
@@ -282,11 +284,11 @@ __device__ __forceinline__ int run_first_strip(int r0, const dev::lds_u32* prof,
         return run_strip<G, RMAX, false, STORE>(prof, t, line, wchunks, k);
     } else {
         if (r0 == R) return run_strip<G, R, false, STORE>(prof, t, line, wchunks, k);
-        return run_first_strip<G, RMAX, STORE, R + kRStep>(r0, prof, t, line, wchunks, k);
+        return run_first_strip<G, RMAX, STORE, R + kRowStep>(r0, prof, t, line, wchunks, k);
     }
 }
 
-template <int G, int RMAX, int R = kRStep>
+template <int G, int RMAX, int R = kRowStep>
 __device__ __forceinline__ void build_profile_r(int r, dev::lds_u32* prof, const dev::lds_u8* tab, const uint16_t* qseq, int row0,
                                 int pad_top, bool transpose, int lane)
 {
@@ -294,7 +296,7 @@ __device__ __forceinline__ void build_profile_r(int r, dev::lds_u32* prof, const
         build_profile<G, RMAX>(prof, tab, qseq, row0, pad_top, transpose, lane);
     } else {
         if (r == R) build_profile<G, R>(prof, tab, qseq, row0, pad_top, transpose, lane);
-        else build_profile_r<G, RMAX, R + kRStep>(r, prof, tab, qseq, row0, pad_top, transpose, lane);
+        else build_profile_r<G, RMAX, R + kRowStep>(r, prof, tab, qseq, row0, pad_top, transpose, lane);
     }
 }
 
@@ -324,7 +326,7 @@ __device__ __forceinline__ void run_item(const BatchArgs& a, const int4 item, de
     // Strips: a first strip of R0 rows per lane, then full strips of RMAX.
     const int full = (Lq - 1) / STRIP_MAX;            // strips after the first
     const int rem = Lq - full * STRIP_MAX;            // rows of the first strip
-    const int R0 = min(RMAX, ((rem + G * kRStep - 1) / (G * kRStep)) * kRStep);
+    const int R0 = min(RMAX, ((rem + G * kRowStep - 1) / (G * kRowStep)) * kRowStep);
     const int pad_top = G * R0 - rem;
     const int nst = full + 1;
 

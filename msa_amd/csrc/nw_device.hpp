@@ -77,13 +77,17 @@ __device__ __forceinline__ int group_shl(int src)
     else return dpp<kWaveShl1>(src, src);
 }
 
+// Profile words holding R rows, four rows per word.
+template <int R>
+constexpr int prof_words = (R + 3) / 4;
+
 // One DP column segment: R cells of one lane.  h[] holds the lane's rows at
 // the previous column (left neighbours) and receives this column; `up` is the
 // cell above the first row (this column), `diag` the cell above-left.  w[]
 // holds the R profile bytes sxx[row][column code], row r in byte r%4 of word
 // r/4.  In the shifted domain every cell is add + max3.
 template <int R>
-__device__ __forceinline__ void cells(int (&h)[R], int up, int diag, const uint32_t (&w)[R / 4])
+__device__ __forceinline__ void cells(int (&h)[R], int up, int diag, const uint32_t (&w)[prof_words<R>])
 {
     int d = diag;
     int u = up;

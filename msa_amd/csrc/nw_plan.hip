@@ -372,8 +372,9 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
                 out.used |= 1u << cfg;
                 out.need_lines = out.need_lines || lq > tc.lanes * tc.rows;
                 const int32_t flags = (fast[k].query & 1) | (cfg << 1);
-                // Rows computed: the query rounded up to 4 rows per lane.
-                const int64_t rows = ((int64_t)lq + 4 * tc.lanes - 1) / (4 * tc.lanes) * (4 * tc.lanes);
+                // Rows computed: the query rounded up to kRowStep rows per lane.
+                const int64_t step = (int64_t)kRowStep * tc.lanes;
+                const int64_t rows = ((int64_t)lq + step - 1) / step * step;
                 for (size_t s = k; s < e; s += item_targets) {
                     const int32_t cnt = (int32_t)std::min<size_t>(item_targets, e - s);
                     int64_t cost = 0;

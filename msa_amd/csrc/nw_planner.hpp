@@ -31,6 +31,7 @@ struct TileConfig {
 constexpr int kTileConfigCount = 3;
 constexpr TileConfig kTileConfig[kTileConfigCount] = {{16, 32}, {16, 16}, {32, 16}};
 constexpr int kMaxTileBytes = 512;     // largest lanes*rows: one code's profile block
+constexpr int kRowStep = 2;            // rows-per-lane granularity of a query's first strip
 int tile_config_index(int lanes, int rows);   // -1 when not in the table
 
 int batched_threads();

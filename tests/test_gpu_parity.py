@@ -100,7 +100,7 @@ def test_asymmetric_table_keeps_reference_orientation():
 
 @pytest.mark.parametrize("lo,hi", [(1, 130), (120, 400), (380, 700), (500, 1100), (900, 2100)])
 def test_batched_strip_shapes(lo, hi):
-    """Queries of every first-strip height (8..32 rows per lane) and of one to
+    """Queries of many first-strip heights (2..32 rows per lane) and of one to
     five strips, which exercise the strip boundary line in global memory."""
     rng = np.random.default_rng(lo + hi)
     T, p = H.tables()["blosum62"]
@@ -269,12 +269,28 @@ TILES = [(16, 32), (16, 16), (32, 16)]
 @pytest.mark.parametrize("lo,hi", [(1, 140), (100, 520), (250, 1100), (900, 2100)])
 def test_batched_tile_shapes(lanes, rows, lo, hi):
     """Every tile shape of the batched kernel, forced for all pairs: first
-    strips of every height (4..rows rows per lane) and queries of one to
+    strips of many heights (2..rows rows per lane) and queries of one to
     several strips (strip boundary line in global memory)."""
     rng = np.random.default_rng(lanes * 1000 + rows * 100 + lo + hi)
     T, p = H.tables()["blosum62"]
     db = H.random_db(rng, 12, lo, hi)
     assert_oracle(db, T, p, kernel="batched", lanes=lanes, rows=rows)
+
+
+@pytest.mark.parametrize("lanes,rows", TILES)
+def test_batched_every_first_strip_height(lanes, rows):
+    """Every first-strip height (2, 4, .., rows rows per lane) under every
+    tile shape, for queries of one strip and of two: query lengths are picked
+    in each height's range of residues, and every pair is scored."""
+    rng = np.random.default_rng(7 * lanes + rows)
+    T, p = H.tables()["blosum62"]
+    step = 2 * lanes
+    lengths = []
+    for j in range(1, rows // 2 + 1):
+        rem = step * (j - 1) + 1 + int(rng.integers(0, step))
+        lengths += [rem, rem + lanes * rows]
+    codes = [rng.integers(0, 24, n).astype(np.uint8) for n in lengths]
+    assert_oracle(W.pack(codes), T, p, kernel="batched", lanes=lanes, rows=rows)
 
 
 @pytest.mark.parametrize("lanes,rows", TILES)
