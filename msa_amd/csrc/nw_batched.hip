@@ -7,7 +7,8 @@
 // alignment of the two sequences cut at their first padding code: the
 // reference's padding rules (src/pairwise/needleman/impl/sequential.cpp:
 // 40-66) reduce to that when the column sequence holds no padding code before
-// its last residue.  The shorter sequence of a pair is the query; with a
+// its last residue.  Either sequence of a pair can be the query (the planner
+// picks the cheaper orientation); with a
 // non-symmetric table the profile is built from the transposed table for
 // pairs whose query is the reference's column sequence, so every score keeps
 // the reference's orientation (:87-91).
@@ -89,40 +90,43 @@ __device__ __forceinline__ int row_bcast(int v)
     return __builtin_amdgcn_mov_dpp(v, dev::kRowNewBcast0 + U, 0xF, 0xF, false);
 }
 
-// Profile layout inside one code's block of G*W words (W = ceil(R/4) words of
-// four rows per lane): the lanes' 16-byte planes first ([plane][lane][4
-// words]), then an 8-byte plane ([lane][2 words]) when W % 4 >= 2, then a
-// 4-byte plane ([lane][word]) when W is odd.
+// Profile layout inside one code's block: NP = ceil(W/4) planes of 16 bytes
+// per lane ([plane][lane][4 words]; W = ceil(R/4) words of four rows per
+// lane), so that every read of a lane is the lane's 16-byte slot plus a
+// constant, one address per step, and the block stride G*NP*16 bytes is a
+// power of two.  Words of the last plane past W are never read.
+template <int R>
+constexpr int prof_planes = (dev::prof_words<R> + 3) / 4;
+template <int G, int R>
+constexpr int code_words = G * 4 * prof_planes<R>;
+
 template <int G, int R>
 __device__ __forceinline__ void load_prof(uint32_t (&w)[dev::prof_words<R>], const dev::lds_u32* code_base, int k)
 {
     constexpr int W = dev::prof_words<R>;
-    constexpr int F = W / 4, H = (W % 4) / 2, Q = W % 2;
+    constexpr int F = W / 4, REM = W % 4;
+    const dev::lds_u32* slot = code_base + k * 4;
 #pragma unroll
     for (int j = 0; j < F; ++j) {
-        const dev::u32x4 v = *(const dev::lds_u32x4*)(code_base + j * G * 4 + k * 4);
+        const dev::u32x4 v = *(const dev::lds_u32x4*)(slot + j * G * 4);
         w[4 * j + 0] = v.x;
         w[4 * j + 1] = v.y;
         w[4 * j + 2] = v.z;
         w[4 * j + 3] = v.w;
     }
-    if constexpr (H) {
-        const dev::u32x2 v = *(const dev::lds_u32x2*)(code_base + F * G * 4 + k * 2);
+    if constexpr (REM >= 2) {
+        const dev::u32x2 v = *(const dev::lds_u32x2*)(slot + F * G * 4);
         w[4 * F + 0] = v.x;
         w[4 * F + 1] = v.y;
     }
-    if constexpr (Q) w[4 * F + 2 * H] = code_base[F * G * 4 + H * G * 2 + k];
+    if constexpr (REM % 2) w[4 * F + (REM & 2)] = slot[F * G * 4 + (REM & 2)];
 }
 
 // Word index of word `w` of lane `k` inside a code block (inverse of load_prof).
-template <int G, int R>
+template <int G>
 __device__ __forceinline__ int prof_word(int k, int w)
 {
-    constexpr int W = dev::prof_words<R>;
-    constexpr int F = W / 4, H = (W % 4) / 2;
-    if (w < 4 * F) return (w >> 2) * G * 4 + k * 4 + (w & 3);
-    if (H && w < 4 * F + 2) return F * G * 4 + k * 2 + (w - 4 * F);
-    return F * G * 4 + H * G * 2 + k;
+    return (w >> 2) * G * 4 + k * 4 + (w & 3);
 }
 
 // Builds the profile of one strip: rows [row0, row0 + G*R) of the padded
@@ -132,7 +136,7 @@ __device__ __forceinline__ void build_profile(dev::lds_u32* prof, const dev::lds
                               const uint16_t* __restrict__ qseq, int row0, int pad_top, bool transpose, int lane)
 {
     constexpr int W = dev::prof_words<R>;
-    constexpr int CODE_WORDS = G * W;
+    constexpr int CODE_WORDS = code_words<G, R>;
     for (int slot = lane; slot < G * W; slot += 64) {
         const int k = slot / W;
         const int w = slot - k * W;
@@ -143,7 +147,7 @@ __device__ __forceinline__ void build_profile(dev::lds_u32* prof, const dev::lds
             const int code = (q >= 0 && 4 * w + t < R) ? dev::code_at(qseq, q) : -1;
             codes[t] = code < kNull ? code : -1;
         }
-        dev::lds_u32* dst = prof + prof_word<G, R>(k, w);
+        dev::lds_u32* dst = prof + prof_word<G>(k, w);
         for (int b = 0; b < kCodes; ++b) {
             uint32_t word = 0;
 #pragma unroll
@@ -197,8 +201,9 @@ __device__ __forceinline__ int run_strip(const dev::lds_u32* prof, const TargetS
                          int wchunks, int k)
 {
     static_assert(G == 16 || G == 32, "lane groups of 16 or 32");
+    static_assert(code_words<G, R> * 4 <= kMaxTileBytes, "profile block exceeds the wave's region");
     constexpr int W = dev::prof_words<R>;
-    constexpr int CODE_WORDS = G * W;
+    constexpr int CODE_WORDS = code_words<G, R>;
     int h[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) h[r] = 0;

@@ -17,7 +17,10 @@
 //   - long kernel: other pairs whose shorter side reaches the long threshold;
 //   - batched kernel: the rest.  For these pairs the reference's rules reduce
 //     to the global alignment of the two sequences cut at their first padding
-//     code; the shorter one is the query.
+//     code, so either one can be the query (rows; a transposed profile keeps
+//     the reference's orientation): the planner takes the orientation the
+//     batched kernel runs faster.  The shorter side is the query of a long
+//     pair.
 #include "nw_planner.hpp"
 
 #include <algorithm>
@@ -34,6 +37,26 @@ namespace msa {
 namespace {
 
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+// Estimated lane instructions of the batched kernel for one pair: a query of
+// lq rows against a target of lt columns under tile (G lanes, at most RMAX
+// rows per lane).  Each strip walks ceil((lt + 2G - 2) / G) * G steps of 2R
+// cell operations plus about 5 instructions (one more when the strip stores
+// its bottom row, and one more when it loads the one above), on G lanes.  A
+// short query fills a lane's rows poorly; a short target pays each strip's
+// G - 1 steps of skew.
+int64_t batched_pair_cost(int64_t lq, int64_t lt, const TileConfig& tc)
+{
+    const int64_t g = tc.lanes, rmax = tc.rows;
+    const int64_t full = (lq - 1) / (g * rmax);
+    const int64_t rem = lq - full * g * rmax;
+    const int64_t step = (int64_t)kRowStep * g;
+    const int64_t r0 = std::min(rmax, (rem + step - 1) / step * kRowStep);
+    const int64_t steps = (lt + 2 * g - 2) / g * g;
+    int64_t cost = steps * (2 * r0 + 5 + (full > 0 ? 1 : 0));
+    if (full > 0) cost += steps * (full * (2 * rmax + 6) + (full - 1));
+    return cost * g;
+}
 
 // Host threads for planning `work` pair visits: one below 2^20 (serving-size
 // jobs plan on their submitting thread), else up to 16.
@@ -198,6 +221,10 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
         return seqs[a].first_end != seqs[b].first_end ? seqs[a].first_end > seqs[b].first_end : a > b;
     });
     for (int32_t r = 0; r < nseq; ++r) rank[by_len[r]] = r;
+    // Tile shape of each sequence as a batched query.
+    std::vector<uint8_t> qcfg(nseq);
+    for (int32_t s = 0; s < nseq; ++s)
+        qcfg[s] = (uint8_t)(forced >= 0 ? forced : tuning_config_for(*tuning, seqs[s].first_end));
     // Routing of one pair (first, second) as the reference orders it.
     enum { kFast, kLong, kPlain };
     struct Route { int32_t kind, qkey, target; int64_t cells; };
@@ -209,11 +236,16 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
         const int64_t c = (int64_t)r.first_end * col.last_real;
         const bool clean = col.first_end == col.last_real && r.first_end > 0 && col.last_real > 0;
         if (kernel == MSA_KERNEL_PLAIN || !clean) return {kPlain, 0, 0, c};
-        const bool row_query = r.first_end < col.last_real || (r.first_end == col.last_real && rowref < colref);
+        const bool is_long = kernel != MSA_KERNEL_BATCHED && std::min(r.first_end, col.last_real) >= long_threshold;
+        bool row_query = r.first_end < col.last_real || (r.first_end == col.last_real && rowref < colref);
+        if (!is_long) {
+            const int64_t as_row = batched_pair_cost(r.first_end, col.last_real, kTileConfig[qcfg[rowref]]);
+            const int64_t as_col = batched_pair_cost(col.last_real, r.first_end, kTileConfig[qcfg[colref]]);
+            if (as_row != as_col) row_query = as_row < as_col;
+        }
         const int32_t q = row_query ? rowref : colref;
         const int32_t t = row_query ? colref : rowref;
         const int32_t transpose = (!symmetric && !row_query) ? 1 : 0;
-        const bool is_long = kernel != MSA_KERNEL_BATCHED && std::min(r.first_end, col.last_real) >= long_threshold;
         return {is_long ? kLong : kFast, q * 2 + transpose, t, c};
     };
     auto add_plain = [](Part& P, int32_t first, int32_t second, int64_t idx) {
