@@ -38,25 +38,27 @@ namespace {
 
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
-// Estimated lane instructions of the batched kernel for one pair: a query of
-// lq rows against a target of lt columns under tile (G lanes, at most RMAX
-// rows per lane).  Each strip walks ceil((lt + 2G - 2) / G) * G steps of 2R
-// cell operations plus about 5 instructions (one more when the strip stores
-// its bottom row, and one more when it loads the one above), on G lanes.  A
-// short query fills a lane's rows poorly; a short target pays each strip's
-// G - 1 steps of skew.
-int64_t batched_pair_cost(int64_t lq, int64_t lt, const TileConfig& tc)
+// Estimated lane instructions of the batched kernel for one pair, as
+// query_weight(query) * target_steps(target): a query of lq rows under tile
+// (G lanes, at most RMAX rows per lane) runs a first strip of R0 rows per lane
+// and `full` strips of RMAX, each step costing 2R cell operations plus about
+// 5 instructions (one more when the strip stores its bottom row, one more
+// when it loads the one above), on G lanes; every strip walks the target's
+// ceil((lt + 2G - 2) / G) * G steps.  A short query fills a lane's rows
+// poorly; a short target pays each strip's G - 1 steps of skew.
+int64_t query_weight(int64_t lq, const TileConfig& tc)
 {
     const int64_t g = tc.lanes, rmax = tc.rows;
     const int64_t full = (lq - 1) / (g * rmax);
     const int64_t rem = lq - full * g * rmax;
     const int64_t step = (int64_t)kRowStep * g;
     const int64_t r0 = std::min(rmax, (rem + step - 1) / step * kRowStep);
-    const int64_t steps = (lt + 2 * g - 2) / g * g;
-    int64_t cost = steps * (2 * r0 + 5 + (full > 0 ? 1 : 0));
-    if (full > 0) cost += steps * (full * (2 * rmax + 6) + (full - 1));
-    return cost * g;
+    int64_t per_step = 2 * r0 + 5 + (full > 0 ? 1 : 0);
+    if (full > 0) per_step += full * (2 * rmax + 6) + (full - 1);
+    return per_step * g;
 }
+
+int64_t target_steps(int64_t lt, int64_t g) { return (lt + 2 * g - 2) / g * g; }
 
 // Host threads for planning `work` pair visits: one below 2^20 (serving-size
 // jobs plan on their submitting thread), else up to 16.
@@ -221,10 +223,21 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
         return seqs[a].first_end != seqs[b].first_end ? seqs[a].first_end > seqs[b].first_end : a > b;
     });
     for (int32_t r = 0; r < nseq; ++r) rank[by_len[r]] = r;
-    // Tile shape of each sequence as a batched query.
-    std::vector<uint8_t> qcfg(nseq);
-    for (int32_t s = 0; s < nseq; ++s)
-        qcfg[s] = (uint8_t)(forced >= 0 ? forced : tuning_config_for(*tuning, seqs[s].first_end));
+    // Cost terms of each sequence for the orientation of batched pairs: as a
+    // query (weight and lane-group width of its tile) and as a target (steps
+    // under 16- and 32-lane groups).
+    struct CostTerms { int64_t weight; int32_t lanes; int32_t steps16, steps32; };
+    std::vector<CostTerms> cterm(nseq);
+    for (int32_t s = 0; s < nseq; ++s) {
+        const int cfg = forced >= 0 ? forced : tuning_config_for(*tuning, seqs[s].first_end);
+        const int64_t len = seqs[s].first_end;
+        cterm[s] = {query_weight(len, kTileConfig[cfg]), kTileConfig[cfg].lanes, (int32_t)target_steps(len, 16),
+                    (int32_t)target_steps(len, 32)};
+    }
+    auto pair_cost = [&](int32_t q, int32_t t) {
+        const CostTerms& cq = cterm[q];
+        return cq.weight * (cq.lanes == 16 ? cterm[t].steps16 : cterm[t].steps32);
+    };
     // Routing of one pair (first, second) as the reference orders it.
     enum { kFast, kLong, kPlain };
     struct Route { int32_t kind, qkey, target; int64_t cells; };
@@ -239,8 +252,8 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
         const bool is_long = kernel != MSA_KERNEL_BATCHED && std::min(r.first_end, col.last_real) >= long_threshold;
         bool row_query = r.first_end < col.last_real || (r.first_end == col.last_real && rowref < colref);
         if (!is_long) {
-            const int64_t as_row = batched_pair_cost(r.first_end, col.last_real, kTileConfig[qcfg[rowref]]);
-            const int64_t as_col = batched_pair_cost(col.last_real, r.first_end, kTileConfig[qcfg[colref]]);
+            const int64_t as_row = pair_cost(rowref, colref);
+            const int64_t as_col = pair_cost(colref, rowref);
             if (as_row != as_col) row_query = as_row < as_col;
         }
         const int32_t q = row_query ? rowref : colref;
