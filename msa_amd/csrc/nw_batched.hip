@@ -18,12 +18,12 @@
 // lane groups (64/G groups of G lanes) score 64/G targets at a time.  Inside a
 // group, lane k owns R consecutive rows of the strip; the DP runs as an
 // anti-diagonal wavefront -- at step s lane k updates column s-k -- with the
-// column residue and the bottom score of each lane moving one lane down per
-// step by DPP shifts.  The substitution scores of a lane's rows come from the
-// strip's profile in the wave's LDS region: R bytes per (column code, lane),
-// read as 16-, 8- and 4-byte planes at a matching lane stride (conflict-free),
-// one step ahead of use.  A strip's bottom row goes to a per-group line in
-// global memory for the next strip.
+// bottom score of each lane moving one lane down per step by a DPP shift and
+// each lane reading its column's residue code from a small ring in LDS.  The
+// substitution scores of a lane's rows come from the strip's profile in the
+// wave's LDS region: R bytes per (column code, lane) in 16-byte planes
+// (conflict-free), read one step ahead of use.  A strip's bottom row goes to
+// a per-group line in global memory for the next strip.
 //
 // Tile configurations.  Each item carries its tile shape, picked per query
 // length by the planner from the tuning table: G lanes per pair (16 or 32) and
@@ -55,6 +55,8 @@ constexpr int kNull = 24;
 constexpr int kWaves = 4;            // waves per workgroup (independent)
 constexpr int kMaxGroups = 4;        // lane groups per wave at 16 lanes
 constexpr int kProfWords = kCodes * kMaxTileBytes / 4;   // one wave's profile region
+constexpr int kTableBytes = 640;     // the 25x25 sxx table, padded to 16 bytes
+constexpr int kRingBytes = 256;      // one wave's code rings: 4G bytes per lane group
 
 static_assert(kTileConfigCount == 3, "tile table and kernel dispatch must agree");
 
@@ -188,29 +190,43 @@ __device__ __forceinline__ int line_value(const TargetState& t, const int32_t* _
 // left its bottom row in `line`; STORE: this strip leaves its bottom row there.
 // Returns (on the group's last lane) the last row's score at the last column.
 //
-// Per step, each lane needs the code of its column (lane 0 of the group: the
-// next residue of the target; lane k: lane k-1's code of the step before) and
-// the cell above its first row (lane 0: the line value; lane k: lane k-1's
-// bottom cell).  With 16-lane groups (one DPP row each) the next residue is
-// broadcast from the chunk's residue vector with row_newbcast and the row
-// shift takes it as its "old" value, two moves per stream and step.  With
-// 32-lane groups the vector is shifted down the wave instead and the carry
-// into the upper row comes from a row broadcast of lane 15.
+// Per step, each lane needs the code of its column and the cell above its
+// first row.  Codes come from the group's code ring in LDS (4G bytes): at step
+// s lane k works virtual column s-k, and column i sits at byte i mod 2G and
+// again at i mod 2G + 2G, so a lane reads the code for step u of a chunk at a
+// per-chunk base plus u -- one LDS byte read two steps ahead, no VALU work;
+// each chunk's codes go in one chunk ahead.  The reads are volatile so that
+// they stay single-byte reads (merged into wider loads they would cost VALU
+// extracts).  The cell above: lane 0 takes the line value, lane k lane k-1's
+// bottom cell.  With 16-lane groups (one DPP row each) the line value is
+// broadcast from the chunk's line vector with row_newbcast and the row shift
+// takes it as its "old" value; with 32-lane groups the vector is shifted down
+// the wave and the carry into the upper row comes from a row broadcast of
+// lane 15.
 template <int G, int R, bool LOAD, bool STORE>
 __device__ __forceinline__ int run_strip(const dev::lds_u32* prof, const TargetState& t, int32_t* __restrict__ line,
-                         int wchunks, int k)
+                         volatile dev::lds_u8* cring, int wchunks, int k)
 {
     static_assert(G == 16 || G == 32, "lane groups of 16 or 32");
     static_assert(code_words<G, R> * 4 <= kMaxTileBytes, "profile block exceeds the wave's region");
     constexpr int W = dev::prof_words<R>;
     constexpr int CODE_WORDS = code_words<G, R>;
+    constexpr int RING = 2 * G;
     int h[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) h[r] = 0;
     int dtop = 0;
     int result = 0;
 
-    int cc = col_code<G>(t, 0, k);
+    // Code ring: virtual column i at byte i mod 2G and again at i mod 2G + 2G;
+    // columns -G..-1 are neutral, chunk 0 goes in before the first read.
+    cring[G + k] = kNull;
+    cring[3 * G + k] = kNull;
+    {
+        const int c0 = col_code<G>(t, 0, k);
+        cring[k] = c0;
+        cring[RING + k] = c0;
+    }
     int cc_next = col_code<G>(t, 1, k);
     int ln = 0, ln_next = 0;
     if constexpr (LOAD) {
@@ -219,25 +235,25 @@ __device__ __forceinline__ int run_strip(const dev::lds_u32* prof, const TargetS
     }
     int outv = 0;
 
-    // b(0): lane 0 takes the first residue, the other lanes start neutral.
-    int b = dev::group_shr<G>(cc, kNull);
+    // Codes of steps 0 and 1 (virtual columns -k and 1-k).
+    const int base0 = (RING - k) & (RING - 1);
+    int b = cring[base0];
+    int bn = cring[base0 + 1];
     uint32_t w[W];
     load_prof<G, R>(w, prof + b * CODE_WORDS, k);
 
     for (int c = 0; c < wchunks; ++c) {
+        // Byte of the code for step u + 2 of this chunk: base + u.
+        const int base = (c * G + 2 - k) & (RING - 1);
         static_for<0, G>([&](auto uc) {
             constexpr int u = decltype(uc)::value;
-            // Code stream one step ahead: lane 0 takes the next residue.
-            int bn;
-            if constexpr (G == 16) {
-                int first;
-                if constexpr (u < G - 1) first = row_bcast<u + 1>(cc);
-                else first = row_bcast<0>(cc_next);
-                bn = dev::dpp<dev::kRowShr1>(first, b);
-            } else {
-                cc = u < G - 1 ? dev::group_shl<G>(cc) : cc_next;
-                bn = dev::group_shr<G>(cc, b);
+            // Chunk c+1 replaces chunk c-1, whose last reader was step G-4.
+            if constexpr (u == G - 3) {
+                const int slot = ((c + 1) * G + k) & (RING - 1);
+                cring[slot] = cc_next;
+                cring[RING + slot] = cc_next;
             }
+            const int bnn = cring[base + u];
             uint32_t wn[W];
             load_prof<G, R>(wn, prof + bn * CODE_WORDS, k);
 
@@ -268,10 +284,10 @@ __device__ __forceinline__ int run_strip(const dev::lds_u32* prof, const TargetS
                 }
             }
             b = bn;
+            bn = bnn;
 #pragma unroll
             for (int q = 0; q < W; ++q) w[q] = wn[q];
         });
-        if constexpr (G == 16) cc = cc_next;
         if constexpr (LOAD) ln = ln_next;
         result = (c == t.chunks - 1) ? h[R - 1] : result;
         cc_next = col_code<G>(t, c + 2, k);
@@ -282,14 +298,14 @@ __device__ __forceinline__ int run_strip(const dev::lds_u32* prof, const TargetS
 
 // First strips: R0 rows per lane, R0 in kRowStep..RMAX.
 template <int G, int RMAX, bool STORE, int R = kRowStep>
-__device__ __forceinline__ int run_first_strip(int r0, const dev::lds_u32* prof, const TargetState& t, int32_t* line, int wchunks,
-                               int k)
+__device__ __forceinline__ int run_first_strip(int r0, const dev::lds_u32* prof, const TargetState& t, int32_t* line,
+                               volatile dev::lds_u8* cring, int wchunks, int k)
 {
     if constexpr (R >= RMAX) {
-        return run_strip<G, RMAX, false, STORE>(prof, t, line, wchunks, k);
+        return run_strip<G, RMAX, false, STORE>(prof, t, line, cring, wchunks, k);
     } else {
-        if (r0 == R) return run_strip<G, R, false, STORE>(prof, t, line, wchunks, k);
-        return run_first_strip<G, RMAX, STORE, R + kRowStep>(r0, prof, t, line, wchunks, k);
+        if (r0 == R) return run_strip<G, R, false, STORE>(prof, t, line, cring, wchunks, k);
+        return run_first_strip<G, RMAX, STORE, R + kRowStep>(r0, prof, t, line, cring, wchunks, k);
     }
 }
 
@@ -317,13 +333,14 @@ __device__ __forceinline__ void wave_lds_sync()
 // All strips of one item with tile shape (G, RMAX).
 template <int G, int RMAX>
 __device__ __forceinline__ void run_item(const BatchArgs& a, const int4 item, dev::lds_u32* prof, const dev::lds_u8* tab,
-                         int32_t* lines, int lane)
+                         dev::lds_u8* rings, int32_t* lines, int lane)
 {
     constexpr int GROUPS = 64 / G;
     constexpr int STRIP_MAX = G * RMAX;
     const int g = lane / G;
     const int k = lane % G;
     int32_t* line = lines + (int64_t)g * a.line_stride;
+    volatile dev::lds_u8* cring = rings + g * 4 * G;
     const SeqInfo qi = a.info[item.x];
     const int Lq = qi.z;
     const uint16_t* qseq = a.blocks + qi.x;
@@ -363,11 +380,11 @@ __device__ __forceinline__ void run_item(const BatchArgs& a, const int4 item, de
             }
             const bool store = st + 1 < nst;
             if (st == 0) {
-                res = store ? run_first_strip<G, RMAX, true>(R0, prof, t, line, wchunks, k)
-                            : run_first_strip<G, RMAX, false>(R0, prof, t, line, wchunks, k);
+                res = store ? run_first_strip<G, RMAX, true>(R0, prof, t, line, cring, wchunks, k)
+                            : run_first_strip<G, RMAX, false>(R0, prof, t, line, cring, wchunks, k);
             } else {
-                res = store ? run_strip<G, RMAX, true, true>(prof, t, line, wchunks, k)
-                            : run_strip<G, RMAX, true, false>(prof, t, line, wchunks, k);
+                res = store ? run_strip<G, RMAX, true, true>(prof, t, line, cring, wchunks, k)
+                            : run_strip<G, RMAX, true, false>(prof, t, line, cring, wchunks, k);
             }
             // The next strip reads this strip's bottom row from `line`.
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -384,6 +401,7 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(3))
     dev::lds_u32* lds_base = (dev::lds_u32*)lds;   // address-space cast: 32-bit LDS offsets
     dev::lds_u32* prof = lds_base + wave * kProfWords;
     dev::lds_u8* tab = (dev::lds_u8*)(lds_base + kWaves * kProfWords);
+    dev::lds_u8* rings = tab + kTableBytes + wave * kRingBytes;
     for (int i = threadIdx.x; i < 640; i += kWaves * 64) tab[i] = a.sxx.v[i];
     __syncthreads();
 
@@ -396,9 +414,9 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(3))
         if (it >= a.nitems) break;
         const int4 item = a.items[it];
         switch ((item.w >> 1) & 3) {
-        case 0: run_item<kTileConfig[0].lanes, kTileConfig[0].rows>(a, item, prof, tab, lines, lane); break;
-        case 1: run_item<kTileConfig[1].lanes, kTileConfig[1].rows>(a, item, prof, tab, lines, lane); break;
-        default: run_item<kTileConfig[2].lanes, kTileConfig[2].rows>(a, item, prof, tab, lines, lane); break;
+        case 0: run_item<kTileConfig[0].lanes, kTileConfig[0].rows>(a, item, prof, tab, rings, lines, lane); break;
+        case 1: run_item<kTileConfig[1].lanes, kTileConfig[1].rows>(a, item, prof, tab, rings, lines, lane); break;
+        default: run_item<kTileConfig[2].lanes, kTileConfig[2].rows>(a, item, prof, tab, rings, lines, lane); break;
         }
     }
 }
@@ -412,7 +430,7 @@ __global__ void queue_reset_kernel(int32_t* queue)
 
 int batched_threads() { return kWaves * 64; }
 
-size_t batched_lds_bytes() { return (size_t)kWaves * kProfWords * 4 + 640; }
+size_t batched_lds_bytes() { return (size_t)kWaves * kProfWords * 4 + kTableBytes + kWaves * kRingBytes; }
 
 int batched_groups_per_block() { return kWaves * kMaxGroups; }
 
