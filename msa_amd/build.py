@@ -21,8 +21,9 @@ INCLUDE = os.path.join(ROOT, "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MSA_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["runtime.hip", "nw_plain.hip", "nw_batched.hip", "nw_long.hip", "nw_plan.hip", "nw_serve.hip",
-               "nw_tune.hip", "nj.hip", "fasta.hip"]
+# Longest compiles first (the batched kernel's three tile configurations).
+HIP_SOURCES = ["nw_batched.hip", "nw_batched_32x16.hip", "nw_batched_16x16.hip", "runtime.hip", "nw_plain.hip",
+               "nw_long.hip", "nw_plan.hip", "nw_serve.hip", "nw_tune.hip", "nj.hip", "fasta.hip"]
 HIP_FLAGS = ["-std=c++17", "-O3", "-fPIC", "--offload-arch=" + ARCH,
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-command-line-argument"]
@@ -33,7 +34,8 @@ def _run(cmd, cwd=None):
     subprocess.run(cmd, check=True, cwd=cwd)
 
 
-def build_hip(jobs=4):
+def build_hip(jobs=None):
+    jobs = jobs or max(1, min(8, os.cpu_count() or 1))
     objdir = os.path.join(PKG, "_build")
     os.makedirs(objdir, exist_ok=True)
     objs = []

@@ -1,0 +1,460 @@
+// The batched Needleman-Wunsch kernel: one persistent launch per tile
+// configuration over a queue of work items, each item one "query" sequence and a run of "targets" (the
+// other sequence of each pair), all sorted by cost by the planner
+// (nw_plan.hip).
+//
+// Which pairs come here.  Only pairs whose score is the plain global
+// alignment of the two sequences cut at their first padding code: the
+// reference's padding rules (src/pairwise/needleman/impl/sequential.cpp:
+// 40-66) reduce to that when the column sequence holds no padding code before
+// its last residue.  Either sequence of a pair can be the query (the planner
+// picks the cheaper orientation); with a
+// non-symmetric table the profile is built from the transposed table for
+// pairs whose query is the reference's column sequence, so every score keeps
+// the reference's orientation (:87-91).
+//
+// Shape of the computation.  Every wave of a workgroup works alone: it takes
+// an item from an atomic counter, splits its query into row strips, and its
+// lane groups (64/G groups of G lanes) score 64/G targets at a time.  Inside a
+// group, lane k owns R consecutive rows of the strip; the DP runs as an
+// anti-diagonal wavefront -- at step s lane k updates column s-k -- with the
+// bottom score of each lane moving one lane down per step by a DPP shift and
+// each lane reading its column's residue code from a small ring in LDS.  The
+// substitution scores of a lane's rows come from the strip's profile in the
+// wave's LDS region: R bytes per (column code, lane) in 16-byte planes
+// (conflict-free), read one step ahead of use.  A strip's bottom row goes to
+// a per-group line in global memory for the next strip.
+//
+// Tile configurations.  Each item carries its tile shape, picked per query
+// length by the planner from the tuning table: G lanes per pair (16 or 32) and
+// at most RMAX rows per lane (16 or 32), so a full strip is G*RMAX rows.  The
+// first strip of a query takes R0 rows per lane, R0 a multiple of kRowStep (2),
+// so at most 2*G - 1 padding rows are computed per query.
+//
+// Neutral padding.  The recurrence runs in the shifted domain G = H +
+// (i+j)*penalty, where every boundary cell is 0 and every cell is
+// max(G_diag + sxx, G_up, G_left) with sxx = T + 2*penalty >= 0.  Rows and
+// columns whose profile entries are 0 and whose neighbours are all 0 stay 0,
+// so the query is padded with such rows at the top of its first strip (the
+// last real row is always the last lane's register R-1 of the last strip) and
+// each target with such columns at its left (every group ends exactly at the
+// end of a G-step chunk).  The inner loop therefore needs no per-step
+// masking: results are captured at chunk ends.
+//
+// Build layout.  The device code is this header; each tile configuration's
+// kernel is instantiated in its own source file (nw_batched.hip for 16 x 32,
+// nw_batched_16x16.hip, nw_batched_32x16.hip), so that the three compile in
+// parallel.
+#pragma once
+
+#include "msa_internal.hpp"
+#include "nw_device.hpp"
+#include "nw_planner.hpp"
+
+#include <type_traits>
+
+namespace msa {
+namespace batched {
+
+
+constexpr int kCodes = 25;           // codes 0..23 and the neutral code 24
+constexpr int kNull = 24;
+constexpr int kWaves = 4;            // waves per workgroup (independent)
+constexpr int kMaxGroups = 4;        // lane groups per wave at 16 lanes
+constexpr int kProfWords = kCodes * kMaxTileBytes / 4;   // one wave's profile region
+constexpr int kTableBytes = 640;     // the 25x25 sxx table, padded to 16 bytes
+constexpr int kRingBytes = 256;      // one wave's code rings: 4G bytes per lane group
+
+// Dynamic LDS of a workgroup: the waves' profile regions, the table, the rings.
+constexpr size_t lds_bytes() { return (size_t)kWaves * kProfWords * 4 + kTableBytes + kWaves * kRingBytes; }
+
+static_assert(kTileConfigCount == 3, "one kernel source per tile configuration");
+
+struct BatchArgs {
+    const uint16_t* blocks;
+    const SeqInfo* info;
+    const int4* items;          // {query, first target, target count, flags}
+    const int2* targets;        // {sequence, output index}
+    int32_t nitems;
+    int32_t* queue;
+    int32_t* out;
+    int32_t* lines;             // one line of line_stride ints per lane group
+    int32_t line_stride;
+    int32_t penalty;
+    Sxx sxx;
+};
+
+// Calls f(integral_constant<int, U>) for U = B..E-1, so that U can name DPP
+// controls, which must be compile-time constants.
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B> {});
+        static_for<B + 1, E>(f);
+    }
+}
+
+// Every lane of each 16-lane row reads lane U of its row.
+template <int U>
+__device__ __forceinline__ int row_bcast(int v)
+{
+    return __builtin_amdgcn_mov_dpp(v, dev::kRowNewBcast0 + U, 0xF, 0xF, false);
+}
+
+// Profile layout inside one code's block: NP = ceil(W/4) planes of 16 bytes
+// per lane ([plane][lane][4 words]; W = ceil(R/4) words of four rows per
+// lane), so that every read of a lane is the lane's 16-byte slot plus a
+// constant, one address per step, and the block stride G*NP*16 bytes is a
+// power of two.  Words of the last plane past W are never read.
+template <int R>
+constexpr int prof_planes = (dev::prof_words<R> + 3) / 4;
+template <int G, int R>
+constexpr int code_words = G * 4 * prof_planes<R>;
+
+template <int G, int R>
+__device__ __forceinline__ void load_prof(uint32_t (&w)[dev::prof_words<R>], const dev::lds_u32* code_base, int k)
+{
+    constexpr int W = dev::prof_words<R>;
+    constexpr int F = W / 4, REM = W % 4;
+    const dev::lds_u32* slot = code_base + k * 4;
+#pragma unroll
+    for (int j = 0; j < F; ++j) {
+        const dev::u32x4 v = *(const dev::lds_u32x4*)(slot + j * G * 4);
+        w[4 * j + 0] = v.x;
+        w[4 * j + 1] = v.y;
+        w[4 * j + 2] = v.z;
+        w[4 * j + 3] = v.w;
+    }
+    if constexpr (REM >= 2) {
+        const dev::u32x2 v = *(const dev::lds_u32x2*)(slot + F * G * 4);
+        w[4 * F + 0] = v.x;
+        w[4 * F + 1] = v.y;
+    }
+    if constexpr (REM % 2) w[4 * F + (REM & 2)] = slot[F * G * 4 + (REM & 2)];
+}
+
+// Word index of word `w` of lane `k` inside a code block (inverse of load_prof).
+template <int G>
+__device__ __forceinline__ int prof_word(int k, int w)
+{
+    return (w >> 2) * G * 4 + k * 4 + (w & 3);
+}
+
+// Builds the profile of one strip: rows [row0, row0 + G*R) of the padded
+// query (rows below pad_top are neutral; bytes past a lane's R rows are 0).
+template <int G, int R>
+__device__ __forceinline__ void build_profile(dev::lds_u32* prof, const dev::lds_u8* tab,
+                              const uint16_t* __restrict__ qseq, int row0, int pad_top, bool transpose, int lane)
+{
+    constexpr int W = dev::prof_words<R>;
+    constexpr int CODE_WORDS = code_words<G, R>;
+    for (int slot = lane; slot < G * W; slot += 64) {
+        const int k = slot / W;
+        const int w = slot - k * W;
+        int codes[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int q = row0 + k * R + 4 * w + t - pad_top;
+            const int code = (q >= 0 && 4 * w + t < R) ? dev::code_at(qseq, q) : -1;
+            codes[t] = code < kNull ? code : -1;
+        }
+        dev::lds_u32* dst = prof + prof_word<G>(k, w);
+        for (int b = 0; b < kCodes; ++b) {
+            uint32_t word = 0;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int code = codes[t];
+                uint32_t v = 0;
+                if (code >= 0 && b < kNull) v = transpose ? tab[b * 25 + code] : tab[code * 25 + b];
+                word |= v << (8 * t);
+            }
+            dst[b * CODE_WORDS] = word;
+        }
+    }
+}
+
+// Per-group view of the target being scored.
+struct TargetState {
+    const uint16_t* seq;
+    int len;        // columns
+    int lpad;       // neutral columns in front, so that len + lpad + G-1 is a multiple of G
+    int chunks;     // (lpad + len + G-1) / G, 0 for an idle group
+};
+
+template <int G>
+__device__ __forceinline__ int col_code(const TargetState& t, int chunk, int k)
+{
+    const int idx = chunk * G + k - t.lpad;
+    return (idx >= 0 && idx < t.len) ? min(dev::code_at(t.seq, idx), kNull) : kNull;
+}
+
+template <int G>
+__device__ __forceinline__ int line_value(const TargetState& t, const int32_t* __restrict__ line, int chunk, int k)
+{
+    const int idx = chunk * G + k - t.lpad;
+    return (idx >= 0 && idx < t.len) ? line[idx] : 0;
+}
+
+// One strip of R rows per lane over the whole target.  LOAD: the strip above
+// left its bottom row in `line`; STORE: this strip leaves its bottom row there.
+// Returns (on the group's last lane) the last row's score at the last column.
+//
+// Per step, each lane needs the code of its column and the cell above its
+// first row.  Codes come from the group's code ring in LDS (4G bytes): at step
+// s lane k works virtual column s-k, and column i sits at byte i mod 2G and
+// again at i mod 2G + 2G, so a lane reads the code for step u of a chunk at a
+// per-chunk base plus u -- one LDS byte read two steps ahead, no VALU work;
+// each chunk's codes go in one chunk ahead.  The reads are volatile so that
+// they stay single-byte reads (merged into wider loads they would cost VALU
+// extracts).  The cell above: lane 0 takes the line value, lane k lane k-1's
+// bottom cell.  With 16-lane groups (one DPP row each) the line value is
+// broadcast from the chunk's line vector with row_newbcast and the row shift
+// takes it as its "old" value; with 32-lane groups the vector is shifted down
+// the wave and the carry into the upper row comes from a row broadcast of
+// lane 15.
+template <int G, int R, bool LOAD, bool STORE>
+__device__ __forceinline__ int run_strip(const dev::lds_u32* prof, const TargetState& t, int32_t* __restrict__ line,
+                         volatile dev::lds_u8* cring, int wchunks, int k)
+{
+    static_assert(G == 16 || G == 32, "lane groups of 16 or 32");
+    static_assert(code_words<G, R> * 4 <= kMaxTileBytes, "profile block exceeds the wave's region");
+    constexpr int W = dev::prof_words<R>;
+    constexpr int CODE_WORDS = code_words<G, R>;
+    constexpr int RING = 2 * G;
+    int h[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) h[r] = 0;
+    int dtop = 0;
+    int result = 0;
+
+    // Code ring: virtual column i at byte i mod 2G and again at i mod 2G + 2G;
+    // columns -G..-1 are neutral, chunk 0 goes in before the first read.
+    cring[G + k] = kNull;
+    cring[3 * G + k] = kNull;
+    {
+        const int c0 = col_code<G>(t, 0, k);
+        cring[k] = c0;
+        cring[RING + k] = c0;
+    }
+    int cc_next = col_code<G>(t, 1, k);
+    int ln = 0, ln_next = 0;
+    if constexpr (LOAD) {
+        ln = line_value<G>(t, line, 0, k);
+        ln_next = line_value<G>(t, line, 1, k);
+    }
+    int outv = 0;
+
+    // Codes of steps 0 and 1 (virtual columns -k and 1-k).
+    const int base0 = (RING - k) & (RING - 1);
+    int b = cring[base0];
+    int bn = cring[base0 + 1];
+    uint32_t w[W];
+    load_prof<G, R>(w, prof + b * CODE_WORDS, k);
+
+    for (int c = 0; c < wchunks; ++c) {
+        // Byte of the code for step u + 2 of this chunk: base + u.
+        const int base = (c * G + 2 - k) & (RING - 1);
+        static_for<0, G>([&](auto uc) {
+            constexpr int u = decltype(uc)::value;
+            // Chunk c+1 replaces chunk c-1, whose last reader was step G-4.
+            if constexpr (u == G - 3) {
+                const int slot = ((c + 1) * G + k) & (RING - 1);
+                cring[slot] = cc_next;
+                cring[RING + slot] = cc_next;
+            }
+            const int bnn = cring[base + u];
+            uint32_t wn[W];
+            load_prof<G, R>(wn, prof + bn * CODE_WORDS, k);
+
+            int hup;
+            if constexpr (LOAD) {
+                if constexpr (G == 16) {
+                    hup = dev::dpp<dev::kRowShr1>(row_bcast<u>(ln), h[R - 1]);
+                } else {
+                    hup = dev::group_shr<G>(ln, h[R - 1]);
+                    if constexpr (u < G - 1) ln = dev::group_shl<G>(ln);
+                }
+            } else {
+                if constexpr (G == 16) hup = dev::dpp_zero<dev::kRowShr1>(h[R - 1]);
+                else hup = dev::group_shr<G>(0, h[R - 1]);
+            }
+
+            dev::cells<R>(h, hup, dtop, w);
+            dtop = hup;
+            // Bottom cells collect in the group's last DPP row: after step u
+            // with u % 16 == 15, its lane j holds the last lane's bottom cell
+            // of step u - 15 + j.
+            if constexpr (STORE) {
+                outv = dev::dpp<dev::kRowShl1>(h[R - 1], outv);
+                if constexpr (u % 16 == 15) {
+                    const int j = k - (G - 16);
+                    const int col = c * G + (u - 15) + j - (G - 1) - t.lpad;
+                    if (j >= 0 && col >= 0 && col < t.len) line[col] = outv;
+                }
+            }
+            b = bn;
+            bn = bnn;
+#pragma unroll
+            for (int q = 0; q < W; ++q) w[q] = wn[q];
+        });
+        if constexpr (LOAD) ln = ln_next;
+        result = (c == t.chunks - 1) ? h[R - 1] : result;
+        cc_next = col_code<G>(t, c + 2, k);
+        if constexpr (LOAD) ln_next = line_value<G>(t, line, c + 2, k);
+    }
+    return result;
+}
+
+// First strips: R0 rows per lane, R0 in kRowStep..RMAX.
+template <int G, int RMAX, bool STORE, int R = kRowStep>
+__device__ __forceinline__ int run_first_strip(int r0, const dev::lds_u32* prof, const TargetState& t, int32_t* line,
+                               volatile dev::lds_u8* cring, int wchunks, int k)
+{
+    if constexpr (R >= RMAX) {
+        return run_strip<G, RMAX, false, STORE>(prof, t, line, cring, wchunks, k);
+    } else {
+        if (r0 == R) return run_strip<G, R, false, STORE>(prof, t, line, cring, wchunks, k);
+        return run_first_strip<G, RMAX, STORE, R + kRowStep>(r0, prof, t, line, cring, wchunks, k);
+    }
+}
+
+template <int G, int RMAX, int R = kRowStep>
+__device__ __forceinline__ void build_profile_r(int r, dev::lds_u32* prof, const dev::lds_u8* tab, const uint16_t* qseq, int row0,
+                                int pad_top, bool transpose, int lane)
+{
+    if constexpr (R >= RMAX) {
+        build_profile<G, RMAX>(prof, tab, qseq, row0, pad_top, transpose, lane);
+    } else {
+        if (r == R) build_profile<G, R>(prof, tab, qseq, row0, pad_top, transpose, lane);
+        else build_profile_r<G, RMAX, R + kRowStep>(r, prof, tab, qseq, row0, pad_top, transpose, lane);
+    }
+}
+
+// LDS accesses of one wave to its own region are ordered; this keeps the
+// compiler from moving them across the profile rebuild and waits for them.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0)
+    __builtin_amdgcn_wave_barrier();
+}
+
+// All strips of one item with tile shape (G, RMAX).
+template <int G, int RMAX>
+__device__ __forceinline__ void run_item(const BatchArgs& a, const int4 item, dev::lds_u32* prof, const dev::lds_u8* tab,
+                         dev::lds_u8* rings, int32_t* lines, int lane)
+{
+    constexpr int GROUPS = 64 / G;
+    constexpr int STRIP_MAX = G * RMAX;
+    const int g = lane / G;
+    const int k = lane % G;
+    int32_t* line = lines + (int64_t)g * a.line_stride;
+    volatile dev::lds_u8* cring = rings + g * 4 * G;
+    const SeqInfo qi = a.info[item.x];
+    const int Lq = qi.z;
+    const uint16_t* qseq = a.blocks + qi.x;
+    const bool transpose = (item.w & 1) != 0;
+    // Strips: a first strip of R0 rows per lane, then full strips of RMAX.
+    const int full = (Lq - 1) / STRIP_MAX;            // strips after the first
+    const int rem = Lq - full * STRIP_MAX;            // rows of the first strip
+    const int R0 = min(RMAX, ((rem + G * kRowStep - 1) / (G * kRowStep)) * kRowStep);
+    const int pad_top = G * R0 - rem;
+    const int nst = full + 1;
+
+    const int rounds = (item.z + GROUPS - 1) / GROUPS;
+    for (int rd = 0; rd < rounds; ++rd) {
+        const int ti = rd * GROUPS + g;
+        TargetState t {a.blocks, 0, 0, 0};
+        int2 tg = make_int2(0, -1);
+        if (ti < item.z) {
+            tg = a.targets[item.y + ti];
+            const SeqInfo si = a.info[tg.x];
+            t.seq = a.blocks + si.x;
+            t.len = si.z;
+            t.chunks = (t.len + 2 * G - 2) / G;
+            t.lpad = t.chunks * G - (t.len + G - 1);
+        }
+        int wchunks = t.chunks;
+        if constexpr (GROUPS > 2) wchunks = max(wchunks, __shfl_xor(wchunks, 16));
+        if constexpr (GROUPS > 1) wchunks = max(wchunks, __shfl_xor(wchunks, 32));
+
+        int res = 0;
+        for (int st = 0; st < nst; ++st) {
+            const int R = st == 0 ? R0 : RMAX;
+            const int row0 = st == 0 ? 0 : G * R0 + (st - 1) * STRIP_MAX;
+            if (nst > 1 || rd == 0) {
+                wave_lds_sync();
+                build_profile_r<G, RMAX>(R, prof, tab, qseq, row0, pad_top, transpose, lane);
+                wave_lds_sync();
+            }
+            const bool store = st + 1 < nst;
+            if (st == 0) {
+                res = store ? run_first_strip<G, RMAX, true>(R0, prof, t, line, cring, wchunks, k)
+                            : run_first_strip<G, RMAX, false>(R0, prof, t, line, cring, wchunks, k);
+            } else {
+                res = store ? run_strip<G, RMAX, true, true>(prof, t, line, cring, wchunks, k)
+                            : run_strip<G, RMAX, true, false>(prof, t, line, cring, wchunks, k);
+            }
+            // The next strip reads this strip's bottom row from `line`.
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        }
+        if (tg.y >= 0 && k == G - 1) a.out[tg.y] = res - (Lq + t.len) * a.penalty;
+    }
+}
+
+template <int CFG>
+__global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(3))) void nw_batched_kernel(BatchArgs a)
+{
+    extern __shared__ __attribute__((aligned(16))) uint32_t lds[];   // 16-byte aligned: ds_read_b128 of profiles
+    const int wave = threadIdx.x / 64;
+    const int lane = threadIdx.x % 64;
+    dev::lds_u32* lds_base = (dev::lds_u32*)lds;   // address-space cast: 32-bit LDS offsets
+    dev::lds_u32* prof = lds_base + wave * kProfWords;
+    dev::lds_u8* tab = (dev::lds_u8*)(lds_base + kWaves * kProfWords);
+    dev::lds_u8* rings = tab + kTableBytes + wave * kRingBytes;
+    for (int i = threadIdx.x; i < 640; i += kWaves * 64) tab[i] = a.sxx.v[i];
+    __syncthreads();
+
+    int32_t* lines = a.lines + (int64_t)(blockIdx.x * kWaves + wave) * kMaxGroups * a.line_stride;
+
+    while (true) {
+        int it = 0;
+        if (lane == 0) it = atomicAdd(a.queue, 1);
+        it = __shfl(it, 0);
+        if (it >= a.nitems) break;
+        run_item<kTileConfig[CFG].lanes, kTileConfig[CFG].rows>(a, a.items[it], prof, tab, rings, lines, lane);
+    }
+}
+
+// Host side of the kernel of tile configuration CFG.
+template <int CFG>
+int kernel_occupancy(int* blocks_per_cu)
+{
+    const void* fn = reinterpret_cast<const void*>(&nw_batched_kernel<CFG>);
+    const size_t lds = lds_bytes();
+    MSA_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int n = 0;
+    MSA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kWaves * 64, lds));
+    *blocks_per_cu = n;
+    return MSA_OK;
+}
+
+template <int CFG>
+int kernel_launch(int grid, const BatchArgs& a, hipStream_t stream)
+{
+    hipLaunchKernelGGL(nw_batched_kernel<CFG>, dim3(grid), dim3(kWaves * 64), lds_bytes(), stream, a);
+    return hip_status(hipGetLastError(), "nw_batched_kernel launch");
+}
+
+// One pair of entry points per tile configuration, each defined in the
+// source file that instantiates its kernel.
+int occupancy_16x32(int* blocks_per_cu);
+int launch_16x32(int grid, const BatchArgs& a, hipStream_t stream);
+int occupancy_16x16(int* blocks_per_cu);
+int launch_16x16(int grid, const BatchArgs& a, hipStream_t stream);
+int occupancy_32x16(int* blocks_per_cu);
+int launch_32x16(int grid, const BatchArgs& a, hipStream_t stream);
+
+}  // namespace batched
+}  // namespace msa

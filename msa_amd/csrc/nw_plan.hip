@@ -445,10 +445,22 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
             need_lines = need_lines || pi.need_lines;
             used |= pi.used;
         }
-        std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) { return x.cost > y.cost; });
+        // One queue per tile configuration (each has its own kernel), every
+        // queue costliest first.
+        std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) {
+            const int cx = (x.v.w >> 1) & 3, cy = (y.v.w >> 1) & 3;
+            return cx != cy ? cx < cy : x.cost > y.cost;
+        });
         hp.items.resize(items.size());
-        for (size_t u = 0; u < items.size(); ++u) hp.items[u] = items[u].v;
-        hp.batch_grid = (int)std::min<int64_t>((int64_t)per_cu * dev.compute_units, ((int64_t)items.size() + 3) / 4);
+        for (size_t u = 0; u < items.size(); ++u) {
+            hp.items[u] = items[u].v;
+            ++hp.batch_first[((items[u].v.w >> 1) & 3) + 1];
+        }
+        for (int c = 0; c < kTileConfigCount; ++c) {
+            const int64_t n = hp.batch_first[c + 1];
+            hp.batch_first[c + 1] += hp.batch_first[c];
+            hp.batch_grid[c] = (int)std::min<int64_t>((int64_t)per_cu * dev.compute_units, (n + 3) / 4);
+        }
         // Lines are touched only by queries of more than one strip.
         hp.batch_stride = need_lines ? ((max_t + 63) & ~63) : 0;
         hp.info.bins = __builtin_popcount(used);
@@ -498,7 +510,9 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
         for (size_t u = 0; u < order.size(); ++u) hp.long_pairs[u] = order[u].second;
         hp.long_grid = (int)std::min<int64_t>((int64_t)per_cu * dev.compute_units, (int64_t)hp.long_pairs.size());
     }
-    hp.info.launches = (int32_t)((hp.items.empty() ? 0 : 2) + (hp.long_pairs.empty() ? 0 : 2) +
+    int batched_launches = 0;
+    for (int c = 0; c < kTileConfigCount; ++c) batched_launches += hp.batch_grid[c] > 0 ? 2 : 0;
+    hp.info.launches = (int32_t)(batched_launches + (hp.long_pairs.empty() ? 0 : 2) +
                                  (hp.plain_index.empty() ? 0 : 1));
     return MSA_OK;
 }
@@ -537,8 +551,12 @@ ScratchNeed scratch_need(const HostPlan& hp)
     ScratchNeed n;
     auto up64 = [](size_t v) { return (v + 63) & ~size_t(63); };
     if (!hp.plain_index.empty()) n.plain_lines = up64((size_t)hp.plain_grid * hp.plain_stride);
-    if (!hp.items.empty())
-        n.batch_lines = up64((size_t)hp.batch_grid * batched_groups_per_block() * hp.batch_stride);
+    if (!hp.items.empty()) {
+        // The configurations' launches run one after another on the stream
+        // and share the lines.
+        const int grid = *std::max_element(hp.batch_grid, hp.batch_grid + kTileConfigCount);
+        n.batch_lines = up64((size_t)grid * batched_groups_per_block() * hp.batch_stride);
+    }
     if (!hp.long_pairs.empty()) n.long_lines = up64((size_t)hp.long_grid * hp.long_boundaries * hp.long_stride);
     return n;
 }
@@ -564,11 +582,14 @@ int launch_plan(const HostPlan& hp, const DataLayout& lay, const void* d_data, c
     const char* base = static_cast<const char*>(d_data);
     const uint16_t* d_blocks = reinterpret_cast<const uint16_t*>(base + lay.blocks);
     const SeqInfo* d_info = reinterpret_cast<const SeqInfo*>(base + lay.seqinfo);
-    if (!hp.items.empty())
-        MSA_TRY(batched_launch(hp.batch_grid, d_blocks, d_info, reinterpret_cast<const int4*>(base + lay.items),
-                               (int32_t)hp.items.size(), reinterpret_cast<const int2*>(base + lay.targets),
-                               scr.batch_queue, d_out, scr.batch_lines, std::max(hp.batch_stride, 1), hp.sxx,
-                               hp.penalty, stream));
+    for (int c = 0; c < kTileConfigCount; ++c) {
+        const int64_t first = hp.batch_first[c], n = hp.batch_first[c + 1] - first;
+        if (n == 0) continue;
+        MSA_TRY(batched_launch(c, hp.batch_grid[c], d_blocks, d_info,
+                               reinterpret_cast<const int4*>(base + lay.items) + first, (int32_t)n,
+                               reinterpret_cast<const int2*>(base + lay.targets), scr.batch_queue + 16 * c, d_out,
+                               scr.batch_lines, std::max(hp.batch_stride, 1), hp.sxx, hp.penalty, stream));
+    }
     if (!hp.long_pairs.empty())
         MSA_TRY(long_launch(hp.long_rows, hp.long_grid, hp.long_lds, d_blocks, d_info,
                             reinterpret_cast<const int4*>(base + lay.long_pairs), (int32_t)hp.long_pairs.size(),

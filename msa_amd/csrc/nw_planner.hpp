@@ -38,9 +38,10 @@ int batched_threads();
 size_t batched_lds_bytes();
 int batched_groups_per_block();
 int batched_occupancy(int* blocks_per_cu);
-int batched_launch(int grid, const uint16_t* d_blocks, const SeqInfo* d_info, const int4* d_items, int32_t nitems,
-                   const int2* d_targets, int32_t* d_queue, int32_t* d_out, int32_t* d_lines, int32_t line_stride,
-                   const Sxx& sxx, int32_t penalty, hipStream_t stream);
+// One persistent launch of tile configuration `config` over d_items[0, nitems).
+int batched_launch(int config, int grid, const uint16_t* d_blocks, const SeqInfo* d_info, const int4* d_items,
+                   int32_t nitems, const int2* d_targets, int32_t* d_queue, int32_t* d_out, int32_t* d_lines,
+                   int32_t line_stride, const Sxx& sxx, int32_t penalty, hipStream_t stream);
 int long_rows_for_slots(int slots);
 int long_strip_rows(int rows);
 size_t long_lds_bytes(int rows, int slots, int max_boundaries);
@@ -100,9 +101,10 @@ struct HostPlan {
     int plain_grid = 0;
     int32_t plain_stride = 0;             // ints per boundary line
     // batched kernel
-    std::vector<int4> items;
+    std::vector<int4> items;              // grouped by tile configuration, costliest first in each
     std::vector<int2> targets;
-    int batch_grid = 0;
+    int64_t batch_first[kTileConfigCount + 1] = {};   // items of configuration c: [first[c], first[c+1])
+    int batch_grid[kTileConfigCount] = {};
     int32_t batch_stride = 0;
     // long kernel
     std::vector<int4> long_pairs;

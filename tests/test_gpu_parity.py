@@ -304,6 +304,32 @@ def test_batched_tile_shapes_asymmetric(lanes, rows):
     assert_oracle(db, Ta, p, kernel="batched", lanes=lanes, rows=rows)
 
 
+def test_plan_mixing_every_tile_shape(tmp_path):
+    """A tuning table that gives each query-length bin another tile shape: the
+    plan launches one kernel per shape (each with its own queue), and every
+    score, under an asymmetric table too, matches the oracle."""
+    rng = np.random.default_rng(33)
+    T, p = H.tables()["blosum62"]
+    Ta = T.copy()
+    Ta[0, 1], Ta[1, 0], Ta[7, 2], Ta[15, 11] = 6, -2, 3, -4
+    path = str(tmp_path / "mixed.json")
+    with open(path, "w") as f:
+        json.dump({"format": "msa-nw-tuning/1", "long_threshold": 8192,
+                   "bins": [{"min_query": 0, "lanes": 16, "rows": 16},
+                            {"min_query": 200, "lanes": 32, "rows": 16},
+                            {"min_query": 600, "lanes": 16, "rows": 32}]}, f)
+    db = H.random_db(rng, 40, 1, 1500)
+    for table in (T, Ta):
+        assert_oracle(db, table, p, kernel="auto", tuning_file=path)
+    plan = N.Plan(*db, T, p, opts=N.options(N.KERNEL_AUTO, tuning_file=path))
+    try:
+        info = plan.info()
+        assert info.bins == 3 and info.plain_pairs == 0
+        assert info.launches == 6          # queue reset + kernel per shape
+    finally:
+        plan.close()
+
+
 def test_unknown_tile_shape_rejected():
     T, p = H.tables()["blosum62"]
     db = H.strings_db(["ACGT", "ACG"])
