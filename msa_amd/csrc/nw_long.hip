@@ -7,9 +7,10 @@
 // GPU one pair per group.  Here a workgroup of NW waves takes one pair at a
 // time.  The query is cut into strips of 64*R rows (a first, partial strip at
 // the top takes the neutral padding rows); strip s goes to wave s % NW.  Each
-// wave runs its strip as a 64-lane wavefront (lane k owns R rows, column
-// residues and bottom scores move one lane per step by DPP wave shifts) over
-// 64-column chunks, i.e. over a row of 64R x 64 tiles.  The strip below
+// wave runs its strip as a 64-lane wavefront (lane k owns R rows, bottom
+// scores move one lane per step by DPP wave shifts, each lane reads its
+// column's slot from a ring in LDS) over 64-column chunks, i.e. over a row of
+// 64R x 64 tiles.  The strip below
 // follows a few tiles behind: after every tile the upper wave stores the
 // strip's bottom row for those 64 columns to a line in global memory (the
 // int32 tile boundary) and publishes its progress in an LDS counter; the
@@ -34,6 +35,7 @@ namespace msa {
 namespace {
 
 constexpr int kLanes = 64;
+constexpr int kRingBytes = 4 * kLanes;   // one wave's slot ring
 constexpr int kSpinLimit = 1 << 24;     // ~1e9 cycles of waiting before giving up
 
 struct LongArgs {
@@ -147,13 +149,20 @@ __device__ __forceinline__ bool wait_for(int* progress, int need, int32_t* error
     return true;
 }
 
+// Column slots come from the wave's slot ring in LDS (256 bytes): at step s
+// lane k works virtual column s-k, and column i sits at byte i mod 128 and
+// again at i mod 128 + 128, so a lane reads the slot for step u of a chunk at
+// a per-chunk base plus u -- one LDS byte read two steps ahead instead of two
+// DPP moves; each chunk's slots go in before step 61 of the chunk before.
+// The reads are volatile so that they stay single-byte reads.
 template <int R, bool LOAD, bool STORE>
 __device__ int run_strip(const dev::lds_u32* prof, const dev::lds_u8* remap, int null_slot, const Target& t,
                          const int32_t* line_in, int* progress_in, int32_t* line_out, int* progress_out,
-                         int32_t* error, int k)
+                         int32_t* error, volatile dev::lds_u8* ring, int k)
 {
     constexpr int W = R / 4;
     constexpr int SLOT_WORDS = kLanes * W;
+    constexpr int RING = 2 * kLanes;
     int h[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) h[r] = 0;
@@ -161,7 +170,14 @@ __device__ int run_strip(const dev::lds_u32* prof, const dev::lds_u8* remap, int
     bool ok = true;
 
     if constexpr (LOAD) ok = wait_for(progress_in, min(t.len, 2 * kLanes - t.lpad), error);
-    int cc = col_slot(t, remap, null_slot, 0, k);
+    // Columns -64..-1 are neutral; chunk 0 goes in before the first read.
+    ring[kLanes + k] = null_slot;
+    ring[3 * kLanes + k] = null_slot;
+    {
+        const int c0 = col_slot(t, remap, null_slot, 0, k);
+        ring[k] = c0;
+        ring[RING + k] = c0;
+    }
     int cc_next = col_slot(t, remap, null_slot, 1, k);
     int ln = 0, ln_next = 0;
     if constexpr (LOAD) {
@@ -169,15 +185,18 @@ __device__ int run_strip(const dev::lds_u32* prof, const dev::lds_u8* remap, int
         ln_next = line_value(t, line_in, 1, k);
     }
     int outv = 0;
-    int b = dev::dpp<dev::kWaveShr1>(cc, null_slot);
+    // Slots of steps 0 and 1 (virtual columns -k and 1-k).
+    const int base0 = (RING - k) & (RING - 1);
+    int b = ring[base0];
+    int bn = ring[base0 + 1];
     uint32_t w[W];
     load_prof<R>(w, prof + b * SLOT_WORDS, k);
 
-    // One wavefront step; LAST: the step that moves to the next chunk.
-    auto step = [&](auto last_tag) {
+    // Step u of a chunk whose slot for step u + 2 sits at byte base + u; LAST:
+    // the step that moves to the next chunk.
+    auto step = [&](auto last_tag, int base, int u) {
         constexpr bool LAST = decltype(last_tag)::value;
-        cc = LAST ? cc_next : dev::dpp<dev::kWaveShl1>(cc, cc);
-        const int bn = dev::dpp<dev::kWaveShr1>(cc, b);
+        const int bnn = ring[base + u];
         uint32_t wn[W];
         load_prof<R>(wn, prof + bn * SLOT_WORDS, k);
         const int hup = LOAD ? dev::dpp<dev::kWaveShr1>(ln, h[R - 1]) : dev::dpp_zero<dev::kWaveShr1>(h[R - 1]);
@@ -186,14 +205,24 @@ __device__ int run_strip(const dev::lds_u32* prof, const dev::lds_u8* remap, int
         dtop = hup;
         if constexpr (STORE) outv = dev::dpp<dev::kWaveShl1>(h[R - 1], outv);
         b = bn;
+        bn = bnn;
 #pragma unroll
         for (int q = 0; q < W; ++q) w[q] = wn[q];
     };
 
     for (int c = 0; c < t.chunks && ok; ++c) {
+        const int base = (c * kLanes + 2 - k) & (RING - 1);
 #pragma unroll 9
-        for (int u = 0; u < kLanes - 1; ++u) step(std::false_type {});
-        step(std::true_type {});
+        for (int u = 0; u < kLanes - 3; ++u) step(std::false_type {}, base, u);
+        // Chunk c+1 replaces chunk c-1, whose last reader was step 60.
+        {
+            const int slot = ((c + 1) * kLanes + k) & (RING - 1);
+            ring[slot] = cc_next;
+            ring[RING + slot] = cc_next;
+        }
+        step(std::false_type {}, base, kLanes - 3);
+        step(std::false_type {}, base, kLanes - 2);
+        step(std::true_type {}, base, kLanes - 1);
         if constexpr (STORE) {
             // Lane k now holds the bottom score of column c*64 + k - 63 - lpad.
             const int col = c * kLanes + k - (kLanes - 1) - t.lpad;
@@ -226,7 +255,8 @@ __global__ __launch_bounds__(NW * 64) void nw_long_kernel(LongArgs a)
     const int prof_words = a.slots * kLanes * R / 4;
     dev::lds_u32* base = (dev::lds_u32*)lds;
     dev::lds_u32* prof = base + wave * prof_words;
-    dev::lds_u8* tab = (dev::lds_u8*)(base + NW * prof_words);
+    volatile dev::lds_u8* ring = (dev::lds_u8*)(base + NW * prof_words) + wave * kRingBytes;
+    dev::lds_u8* tab = (dev::lds_u8*)(base + NW * prof_words) + NW * kRingBytes;
     dev::lds_u8* remap = tab + 640;
     dev::lds_u8* slot_code = remap + 32;
     int* progress = (int*)(slot_code + 32);      // max_boundaries counters
@@ -272,10 +302,16 @@ __global__ __launch_bounds__(NW * 64) void nw_long_kernel(LongArgs a)
             const int32_t* lin = lines + (int64_t)(st - 1) * a.line_stride;
             int32_t* lout = lines + (int64_t)st * a.line_stride;
             int res;
-            if (!load && !store) res = run_strip<R, false, false>(prof, remap, null_slot, t, lin, progress, lout, progress, a.error, lane);
-            else if (!load) res = run_strip<R, false, true>(prof, remap, null_slot, t, lin, progress, lout, progress + st, a.error, lane);
-            else if (store) res = run_strip<R, true, true>(prof, remap, null_slot, t, lin, progress + st - 1, lout, progress + st, a.error, lane);
-            else res = run_strip<R, true, false>(prof, remap, null_slot, t, lin, progress + st - 1, lout, progress, a.error, lane);
+            if (!load && !store)
+                res = run_strip<R, false, false>(prof, remap, null_slot, t, lin, progress, lout, progress, a.error, ring, lane);
+            else if (!load)
+                res = run_strip<R, false, true>(prof, remap, null_slot, t, lin, progress, lout, progress + st, a.error, ring, lane);
+            else if (store)
+                res = run_strip<R, true, true>(prof, remap, null_slot, t, lin, progress + st - 1, lout, progress + st, a.error,
+                                               ring, lane);
+            else
+                res = run_strip<R, true, false>(prof, remap, null_slot, t, lin, progress + st - 1, lout, progress, a.error,
+                                                ring, lane);
             if (st == nst - 1 && lane == kLanes - 1) s_result = res;
         }
         __syncthreads();
@@ -306,7 +342,7 @@ int long_strip_rows(int rows) { return kLanes * rows; }
 
 size_t long_lds_bytes(int rows, int slots, int max_boundaries)
 {
-    return (size_t)kLongWaves * slots * kLanes * rows + 640 + 64 + 4 * (size_t)max_boundaries;
+    return (size_t)kLongWaves * (slots * kLanes * rows + kRingBytes) + 640 + 64 + 4 * (size_t)max_boundaries;
 }
 
 static const void* long_kernel_fn(int rows)
