@@ -331,7 +331,10 @@ int tune(const uint16_t* blocks, const int64_t* offset, const int32_t* size, int
     bool symmetric = true;
     MSA_TRY(msa::make_sxx(table625, penalty, sxx, symmetric));
 
-    // Query lengths: the shorter side of each pair (first padding code).
+    // Query lengths: the shorter side of each pair (first padding code).  The
+    // planner may run a batched pair with its longer side as the query; the
+    // trials time whole executions, so that changes only which bin a pair's
+    // time is charged to.
     auto len = [&](int32_t s) { return std::min(seqs[s].first_end, seqs[s].last_real); };
     std::vector<int32_t> order((size_t)nseq);
     for (int32_t s = 0; s < nseq; ++s) order[(size_t)s] = s;
