@@ -1,17 +1,16 @@
 // The batched Needleman-Wunsch kernel: one persistent launch per tile
-// configuration over a queue of work items, each item one "query" sequence and a run of "targets" (the
-// other sequence of each pair), all sorted by cost by the planner
-// (nw_plan.hip).
+// configuration over a queue of work items, each item one "query" sequence and
+// a run of "targets" (the other sequence of each pair), sorted by cost by the
+// planner (nw_plan.hip).
 //
 // Which pairs come here.  Only pairs whose score is the plain global
 // alignment of the two sequences cut at their first padding code: the
 // reference's padding rules (src/pairwise/needleman/impl/sequential.cpp:
 // 40-66) reduce to that when the column sequence holds no padding code before
 // its last residue.  Either sequence of a pair can be the query (the planner
-// picks the cheaper orientation); with a
-// non-symmetric table the profile is built from the transposed table for
-// pairs whose query is the reference's column sequence, so every score keeps
-// the reference's orientation (:87-91).
+// picks the cheaper orientation); with a non-symmetric table the profile is
+// built from the transposed table for pairs whose query is the reference's
+// column sequence, so every score keeps the reference's orientation (:87-91).
 //
 // Shape of the computation.  Every wave of a workgroup works alone: it takes
 // an item from an atomic counter, splits its query into row strips, and its
@@ -25,8 +24,9 @@
 // (conflict-free), read one step ahead of use.  A strip's bottom row goes to
 // a per-group line in global memory for the next strip.
 //
-// Tile configurations.  Each item carries its tile shape, picked per query
-// length by the planner from the tuning table: G lanes per pair (16 or 32) and
+// Tile configurations.  Each query's tile shape is picked per query length by
+// the planner from the tuning table, and each shape has its own kernel and
+// queue: G lanes per pair (16 or 32) and
 // at most RMAX rows per lane (16 or 32), so a full strip is G*RMAX rows.  The
 // first strip of a query takes R0 rows per lane, R0 a multiple of kRowStep (2),
 // so at most 2*G - 1 padding rows are computed per query.
@@ -55,7 +55,6 @@
 
 namespace msa {
 namespace batched {
-
 
 constexpr int kCodes = 25;           // codes 0..23 and the neutral code 24
 constexpr int kNull = 24;
