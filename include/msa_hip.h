@@ -100,6 +100,21 @@ int msa_event_record(msa_event event, msa_stream stream);
 int msa_event_sync(msa_event event);
 int msa_event_elapsed_ms(msa_event start, msa_event stop, float* ms);
 
+/* ---- graphs ------------------------------------------------------------------
+ * No counterpart in the reference (which launches on the default stream and
+ * synchronises with copies, src/cuda.cuh:279-330); the hot path's own answer
+ * to launch cost is msa_nw_options.use_graph.  These capture any sequence of
+ * this library's asynchronous calls on a created stream (not the null
+ * stream) -- for instance msa_nw_plan_execute of a plan built with
+ * use_graph = 0 -- into a graph that is replayed with one launch.  Between
+ * capture_begin and capture_end nothing on that stream runs; calls that
+ * synchronise or allocate are not capturable and return an error. */
+typedef struct msa_graph_opaque* msa_graph;
+int msa_graph_capture_begin(msa_stream stream);
+int msa_graph_capture_end(msa_stream stream, msa_graph* graph);   /* ends the capture and instantiates */
+int msa_graph_launch(msa_graph graph, msa_stream stream);
+int msa_graph_destroy(msa_graph graph);
+
 /* ---- scoring-table registry (scoring_table::has/make/list,
  *      src/pairwise/table.cu:254-330) ---------------------------------------- */
 int msa_table_count(void);

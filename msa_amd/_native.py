@@ -112,6 +112,10 @@ SIGNATURES = {
     "msa_event_record": (ctypes.c_int, [_vp, _vp]),
     "msa_event_sync": (ctypes.c_int, [_vp]),
     "msa_event_elapsed_ms": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_float)]),
+    "msa_graph_capture_begin": (ctypes.c_int, [_vp]),
+    "msa_graph_capture_end": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "msa_graph_launch": (ctypes.c_int, [_vp, _vp]),
+    "msa_graph_destroy": (ctypes.c_int, [_vp]),
     "msa_table_count": (ctypes.c_int, []),
     "msa_table_name": (ctypes.c_char_p, [ctypes.c_int]),
     "msa_table_get": (ctypes.c_int, [ctypes.c_char_p, _vp, ctypes.POINTER(_i32)]),

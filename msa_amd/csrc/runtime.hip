@@ -7,6 +7,8 @@
 // is checked and reported as a status code.
 #include "msa_internal.hpp"
 
+#include <new>
+
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -215,6 +217,61 @@ int msa_event_elapsed_ms(msa_event start, msa_event stop, float* ms)
 {
     if (!ms) return fail(MSA_ERR_INVALID_ARGUMENT, "ms is null");
     return hip_status(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop), "hipEventElapsedTime");
+}
+
+struct msa_graph_opaque {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int msa_graph_capture_begin(msa_stream stream)
+{
+    if (!stream) return fail(MSA_ERR_INVALID_ARGUMENT, "graphs are captured on a created stream, not the null stream");
+    return hip_status(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal),
+                      "hipStreamBeginCapture");
+}
+
+int msa_graph_capture_end(msa_stream stream, msa_graph* graph)
+{
+    if (!stream) return fail(MSA_ERR_INVALID_ARGUMENT, "graphs are captured on a created stream, not the null stream");
+    if (!graph) return fail(MSA_ERR_INVALID_ARGUMENT, "graph is null");
+    *graph = nullptr;
+    hipGraph_t g = nullptr;
+    MSA_HIP(hipStreamEndCapture((hipStream_t)stream, &g));
+    if (!g) return fail(MSA_ERR_RUNTIME, "stream capture produced no graph");
+    auto* out = new (std::nothrow) msa_graph_opaque();
+    if (!out) {
+        (void)hipGraphDestroy(g);
+        return fail(MSA_ERR_OUT_OF_MEMORY, "host allocation failed");
+    }
+    out->graph = g;
+    const int rc = hip_status(hipGraphInstantiate(&out->exec, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+    if (rc != MSA_OK) {
+        (void)hipGraphDestroy(g);
+        delete out;
+        return rc;
+    }
+    *graph = out;
+    return MSA_OK;
+}
+
+int msa_graph_launch(msa_graph graph, msa_stream stream)
+{
+    if (!graph) return fail(MSA_ERR_INVALID_ARGUMENT, "graph is null");
+    return hip_status(hipGraphLaunch(graph->exec, (hipStream_t)stream), "hipGraphLaunch");
+}
+
+int msa_graph_destroy(msa_graph graph)
+{
+    if (!graph) return MSA_OK;
+    int rc = MSA_OK;
+    if (graph->exec) rc = hip_status(hipGraphExecDestroy(graph->exec), "hipGraphExecDestroy");
+    if (graph->graph) {
+        const int r2 = hip_status(hipGraphDestroy(graph->graph), "hipGraphDestroy");
+        if (rc == MSA_OK) rc = r2;
+    }
+    delete graph;
+    return rc;
 }
 
 int msa_table_count(void) { return kTableCount; }

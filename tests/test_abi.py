@@ -35,6 +35,18 @@ def test_library_exports_every_declared_symbol():
     assert set(N.SIGNATURES) == set(names)
 
 
+def test_graph_entry_points_check_arguments_first():
+    """msa_graph_*: the null stream, a null graph and a null output are
+    refused before any device call; destroying no graph is a no-op."""
+    lib = N.lib()
+    g = ctypes.c_void_p()
+    assert lib.msa_graph_capture_begin(None) == N.MSA_ERR_INVALID_ARGUMENT
+    assert lib.msa_graph_capture_end(None, ctypes.byref(g)) == N.MSA_ERR_INVALID_ARGUMENT
+    assert lib.msa_graph_capture_end(ctypes.c_void_p(1), None) == N.MSA_ERR_INVALID_ARGUMENT
+    assert lib.msa_graph_launch(None, None) == N.MSA_ERR_INVALID_ARGUMENT
+    assert lib.msa_graph_destroy(None) == 0
+
+
 def test_error_strings():
     lib = N.lib()
     for code in range(0, 11):

@@ -6,6 +6,7 @@ full-size workloads are checked through properties that do not depend on a
 CPU pass over all pairs: batched == plain on every pair, a random oracle
 sample, and symmetry.  The bar is bit-exact int32 == the reference's float.
 """
+import ctypes
 import json
 import zlib
 
@@ -172,6 +173,47 @@ def test_graph_replays_match_eager():
         assert np.array_equal(out_g.download(np.int32, n), ref)
         eager.execute(out_e.ptr)
         N.check(N.lib().msa_device_sync())
+
+
+def test_caller_captured_graph_of_a_plan():
+    """msa_graph_*: the caller captures msa_nw_plan_execute of a plan built
+    with use_graph = 0 on its own stream and replays the graph; every replay,
+    with eager executions and synchronisations between, equals the eager
+    scores (plain, batched and long pairs in one plan)."""
+    lib = N.lib()
+    rng = np.random.default_rng(8)
+    T, p = H.tables()["blosum62"]
+    db = H.random_db(rng, 30, 5, 1400, dirty=True)
+    n = 30 * 29 // 2
+    stream = ctypes.c_void_p()
+    N.check(lib.msa_stream_create(ctypes.byref(stream)))
+    plan = N.Plan(*db, T, p, opts=N.options(N.KERNEL_AUTO, use_graph=False, long_threshold=1000))
+    out_e, out_g = N.DeviceBuffer(4 * n), N.DeviceBuffer(4 * n)
+    graph = ctypes.c_void_p()
+    try:
+        info = plan.info()
+        assert info.plain_pairs > 0 and info.batched_pairs > 0 and info.long_pairs > 0
+        plan.execute(out_e.ptr, stream.value)
+        N.check(lib.msa_stream_sync(stream))
+        ref = out_e.download(np.int32, n)
+        assert np.array_equal(ref.astype(np.float32), H.oracle_scores(db, T, p))
+        N.check(lib.msa_graph_capture_begin(stream))
+        plan.execute(out_g.ptr, stream.value)
+        N.check(lib.msa_graph_capture_end(stream, ctypes.byref(graph)))
+        for _ in range(4):
+            N.check(lib.msa_memset(out_g.ptr, 0xAB, 4 * n))
+            N.check(lib.msa_graph_launch(graph, stream))
+            N.check(lib.msa_stream_sync(stream))
+            assert np.array_equal(out_g.download(np.int32, n), ref)
+            plan.execute(out_e.ptr, stream.value)
+            N.check(lib.msa_stream_sync(stream))
+            assert np.array_equal(out_e.download(np.int32, n), ref)
+    finally:
+        N.check(lib.msa_graph_destroy(graph))
+        plan.close()
+        out_e.close()
+        out_g.close()
+        N.check(lib.msa_stream_destroy(stream))
 
 
 def test_plain_entry_on_device_memory():
