@@ -173,7 +173,7 @@ def cpu_baseline(w, seconds, workers, seed=12):
                       "on contiguous shares as the reference's MPI ranks take them" % (n, cells, dt)}
 
 
-def serving_run(N, tables, rates, seconds, pool_size=48, slots=4, seed=5, log_fn=log):
+def serving_run(N, tables, rates, seconds, pool_size=48, slots=4, seed=5, log_fn=log, jobs=0):
     """Poisson job streams through the serving loop.
 
     Jobs: 8..256 protein sequences (count uniform), lengths uniform in
@@ -181,6 +181,7 @@ def serving_run(N, tables, rates, seconds, pool_size=48, slots=4, seed=5, log_fn
     `pool_size` jobs is cycled.  Arrivals: Poisson at each rate (seed 50).
     Latency = scores on the host - arrival (submit call).  Every job's scores
     are compared with the same job scored alone through msa_nw_align.
+    `jobs` > 0 fixes the number of jobs per rate instead of `seconds`.
     """
     import threading
     from msa_amd import workloads as W
@@ -189,7 +190,7 @@ def serving_run(N, tables, rates, seconds, pool_size=48, slots=4, seed=5, log_fn
     alone = [N.align(j.blocks, j.offsets, j.sizes, T, p) for j in pool]
     out = []
     for rate in rates:
-        count = max(8, int(rate * seconds))
+        count = jobs if jobs > 0 else max(8, int(rate * seconds))
         arrivals = W.poisson_arrivals(rate, count, seed=50)
         srv = N.Server(T, p, slots=slots, max_length=2048, device_bytes=4 << 30, host_bytes=512 << 20)
         ids = [None] * count
@@ -401,6 +402,7 @@ def main():
     ap.add_argument("--serving", action="store_true", help="only the serving sweep")
     ap.add_argument("--rates", default="10,20,50,100,200,500")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-jobs", type=int, default=0, help="jobs per rate in the --serving sweep (0: rate x seconds)")
     ap.add_argument("--serve-rate", type=float, default=100.0, help="rate of the serving figure in the main line")
     ap.add_argument("--phylogeny", action="store_true",
                     help="only the neighbor-joining stage over the workload's scores (its own JSON line)")
@@ -422,12 +424,13 @@ def main():
     tables = {n: N.table(n) for n in N.table_names()}
     if args.serving:
         rates = [float(r) for r in args.rates.split(",")]
-        sweep = serving_run(N, tables, rates, args.serve_seconds)
+        sweep = serving_run(N, tables, rates, args.serve_seconds, jobs=args.serve_jobs)
         print(json.dumps({
             "metric": METRIC, "value": max(r["jobs_per_s"] for r in sweep), "unit": "jobs/s",
             "n_gpus": 1, "higher_is_better": True, "data": "synthetic", "dtype": "int32",
             "config": {"workload": "serving: Poisson jobs of 8-256 protein sequences, L~U[100,2000], blosum62",
-                       "seconds_per_rate": args.serve_seconds, "slots": 4},
+                       "seconds_per_rate": None if args.serve_jobs > 0 else args.serve_seconds,
+                       "jobs_per_rate": args.serve_jobs if args.serve_jobs > 0 else None, "slots": 4},
             "serving": sweep}), flush=True)
         return 0 if all(r["bit_exact_vs_alone"] for r in sweep) else 1
     if args.fasta:
