@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <atomic>
 #include <mutex>
 #include <new>
@@ -446,11 +447,31 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
             used |= pi.used;
         }
         // One queue per tile configuration (each has its own kernel), every
-        // queue costliest first.
-        std::stable_sort(items.begin(), items.end(), [](const Item& x, const Item& y) {
+        // queue costliest first.  A stable sort, so the order is unique: the
+        // planning threads sort contiguous ranges, then adjacent ranges are
+        // merged (stably) level by level.
+        auto before = [](const Item& x, const Item& y) {
             const int cx = (x.v.w >> 1) & 3, cy = (y.v.w >> 1) & 3;
             return cx != cy ? cx < cy : x.cost > y.cost;
-        });
+        };
+        const int T = (int)parts.size();
+        if (T > 1) {
+            std::vector<size_t> cut((size_t)T + 1);
+            for (int t = 0; t <= T; ++t) cut[t] = items.size() * (size_t)t / (size_t)T;
+            bool sorted = run_parallel(T, [&](int t) {
+                std::stable_sort(items.begin() + cut[t], items.begin() + cut[t + 1], before);
+            });
+            for (int width = 1; sorted && width < T; width *= 2) {
+                const int merges = (T + 2 * width - 1) / (2 * width);
+                sorted = run_parallel(merges, [&](int m) {
+                    const int a = 2 * width * m, b = std::min(a + width, T), e = std::min(a + 2 * width, T);
+                    if (b < e) std::inplace_merge(items.begin() + cut[a], items.begin() + cut[b], items.begin() + cut[e], before);
+                });
+            }
+            if (!sorted) return fail(MSA_ERR_OUT_OF_MEMORY, "host allocation failed while planning");
+        } else {
+            std::stable_sort(items.begin(), items.end(), before);
+        }
         hp.items.resize(items.size());
         for (size_t u = 0; u < items.size(); ++u) {
             hp.items[u] = items[u].v;
@@ -536,7 +557,20 @@ DataLayout data_layout(const HostPlan& hp)
 void pack_data(const HostPlan& hp, const DataLayout& lay, void* host_block)
 {
     char* base = static_cast<char*>(host_block);
-    auto put = [&](size_t off, const void* src, size_t bytes) { if (bytes) std::memcpy(base + off, src, bytes); };
+    // Copies of 8 MiB and more (the target list of a large plan) run on 8
+    // threads.
+    auto put = [&](size_t off, const void* src, size_t bytes) {
+        if (bytes < (size_t(8) << 20)) {
+            if (bytes) std::memcpy(base + off, src, bytes);
+            return;
+        }
+        const char* from = static_cast<const char*>(src);
+        const bool ok = run_parallel(8, [&](int t) {
+            const size_t a = bytes * (size_t)t / 8, e = bytes * (size_t)(t + 1) / 8;
+            std::memcpy(base + off + a, from + a, e - a);
+        });
+        if (!ok) std::memcpy(base + off, src, bytes);
+    };
     put(lay.blocks, hp.blocks.data(), hp.blocks.size() * sizeof(uint16_t));
     put(lay.seqinfo, hp.seqinfo.data(), hp.seqinfo.size() * sizeof(SeqInfo));
     put(lay.plain_pairs, hp.plain_pairs.data(), hp.plain_pairs.size() * sizeof(int32_t));
@@ -658,13 +692,15 @@ int msa::create_plan(msa_nw_plan* plan, const uint16_t* blocks, const int64_t* o
         if (rc == MSA_OK) {
             p->lay = msa::data_layout(p->hp);
             p->need = msa::scratch_need(p->hp);
-            std::vector<char> staging(p->lay.bytes);
-            msa::pack_data(p->hp, p->lay, staging.data());
+            // pack_data writes every array; the alignment gaps between them
+            // are never read, so the block is not cleared first.
+            std::unique_ptr<char[]> staging(new char[p->lay.bytes]);
+            msa::pack_data(p->hp, p->lay, staging.get());
             rc = msa::hip_status(hipMalloc(&p->d_data, p->lay.bytes), "hipMalloc");
             if (rc == MSA_OK)
                 rc = msa::hip_status(hipMalloc(&p->d_scratch, p->need.words() * sizeof(int32_t)), "hipMalloc");
             if (rc == MSA_OK)
-                rc = msa::hip_status(hipMemcpyAsync(p->d_data, staging.data(), p->lay.bytes, hipMemcpyHostToDevice,
+                rc = msa::hip_status(hipMemcpyAsync(p->d_data, staging.get(), p->lay.bytes, hipMemcpyHostToDevice,
                                                     stream), "hipMemcpyAsync");
             if (rc == MSA_OK) rc = msa::hip_status(hipStreamSynchronize(stream), "hipStreamSynchronize");
             p->scratch = msa::carve_scratch(p->need, p->d_scratch);
