@@ -148,3 +148,29 @@ def test_database_load_on_device_matches_the_host_loader(tmp_path):
     path.write_bytes(text)
     host, dev = Database.load(str(path)), Database.load(str(path), device=True)
     assert host.count == dev.count and _entries(host) == _entries(dev)
+
+
+def test_pipeline_from_fasta_text_to_guide_tree(tmp_path):
+    """The reference's pipeline with every stage on the GPU: the FASTA file
+    parsed on the device (database::load_on_device), all pairs scored by the
+    pairwise module (src/pairwise.cpp:27-55), the tree built by the phylogeny
+    module (src/phylogeny.cpp:25-47).  Against the oracles end to end: the
+    oracle's store of the same text, the oracle's scores of that store, the
+    oracle's neighbor joining of those scores."""
+    from msa_amd.museqa import phylogeny as mpg
+    from msa_amd.museqa._core import core
+    from msa_amd.museqa.database import Database
+    rng = np.random.default_rng(29)
+    text = H.synthetic_fasta(rng, 48, 20, 600, width=60, crlf=False, blank_every=7, junk=True)
+    path = tmp_path / "pipeline.fasta"
+    path.write_bytes(text)
+    db = Database.load(str(path), device=True)
+    out_db, dm, count = core.module_run({"scoring-table": "blosum62"}, db.native)
+    blocks, offset, size, _, _ = ORACLE.parse(text)
+    assert count == len(size) == 48
+    T, p = H.tables()["blosum62"]
+    scores = H.oracle_scores((blocks, offset, size), T, p)
+    got_scores = np.array([dm.at(i, j) for i in range(count) for j in range(i)], dtype=np.float32)
+    assert np.array_equal(got_scores, scores)
+    tdb, tree = core.phylogeny_module_run({"phylogeny": "default"}, out_db, dm, count)
+    H.assert_same_tree(mpg.GuideTree(tree).nodes(), M.NjOracle().build(scores, count, 0))
