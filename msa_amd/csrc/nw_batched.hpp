@@ -22,7 +22,8 @@
 // substitution scores of a lane's rows come from the strip's profile in the
 // wave's LDS region: R bytes per (column code, lane) in 16-byte planes
 // (conflict-free), read one step ahead of use.  A strip's bottom row goes to
-// a per-group line in global memory for the next strip.
+// a per-group line in global memory for the next strip, one buffer store per
+// step from the group's last lane.
 //
 // Tile configurations.  Each query's tile shape is picked per query length by
 // the planner from the tuning table, and each shape has its own kernel and
@@ -173,6 +174,17 @@ __device__ __forceinline__ void build_profile(dev::lds_u32* prof, const dev::lds
     }
 }
 
+// The bottom-row store of a strip: the wave's line regions as one raw buffer
+// (dword 3 of the descriptor: 32-bit data format, as for any raw buffer
+// access on gfx9), and this lane's byte offset into it -- out of range
+// (the store is dropped) on every lane but a group's last.
+constexpr int kRawBufferFlags = 0x00020000;
+constexpr uint32_t kOutOfRange = 0x80000000u;
+struct LineStore {
+    __amdgpu_buffer_rsrc_t rsrc;
+    uint32_t lane_off;
+};
+
 // Per-group view of the target being scored.
 struct TargetState {
     const uint16_t* seq;
@@ -214,7 +226,7 @@ __device__ __forceinline__ int line_value(const TargetState& t, const int32_t* _
 // lane 15.
 template <int G, int R, bool LOAD, bool STORE>
 __device__ __forceinline__ int run_strip(const dev::lds_u32* prof, const TargetState& t, int32_t* __restrict__ line,
-                         volatile dev::lds_u8* cring, int wchunks, int k)
+                         volatile dev::lds_u8* cring, const LineStore& ls, int wchunks, int k)
 {
     static_assert(G == 16 || G == 32, "lane groups of 16 or 32");
     static_assert(code_words<G, R> * 4 <= kMaxTileBytes, "profile block exceeds the wave's region");
@@ -242,7 +254,6 @@ __device__ __forceinline__ int run_strip(const dev::lds_u32* prof, const TargetS
         ln = line_value<G>(t, line, 0, k);
         ln_next = line_value<G>(t, line, 1, k);
     }
-    int outv = 0;
 
     // Codes of steps 0 and 1 (virtual columns -k and 1-k).
     const int base0 = (RING - k) & (RING - 1);
@@ -254,6 +265,10 @@ __device__ __forceinline__ int run_strip(const dev::lds_u32* prof, const TargetS
     for (int c = 0; c < wchunks; ++c) {
         // Byte of the code for step u + 2 of this chunk: base + u.
         const int base = (c * G + 2 - k) & (RING - 1);
+        // Byte offset in the wave's line buffer of the bottom cell the
+        // group's last lane makes at step 0 of this chunk (column c*G - (G-1)
+        // - lpad); far out of range on every other lane.
+        const uint32_t voff = ls.lane_off + (uint32_t)(c * G - (G - 1) - t.lpad) * 4u;
         static_for<0, G>([&](auto uc) {
             constexpr int u = decltype(uc)::value;
             // Chunk c+1 replaces chunk c-1, whose last reader was step G-4.
@@ -281,17 +296,13 @@ __device__ __forceinline__ int run_strip(const dev::lds_u32* prof, const TargetS
 
             dev::cells<R>(h, hup, dtop, w);
             dtop = hup;
-            // Bottom cells collect in the group's last DPP row: after step u
-            // with u % 16 == 15, its lane j holds the last lane's bottom cell
-            // of step u - 15 + j.
-            if constexpr (STORE) {
-                outv = dev::dpp<dev::kRowShl1>(h[R - 1], outv);
-                if constexpr (u % 16 == 15) {
-                    const int j = k - (G - 16);
-                    const int col = c * G + (u - 15) + j - (G - 1) - t.lpad;
-                    if (j >= 0 && col >= 0 && col < t.len) line[col] = outv;
-                }
-            }
+            // The bottom cell goes straight to the line: one buffer store
+            // (the step's 4u in the scalar offset, no VALU work), whose lanes
+            // other than the group's last are out of range and dropped;
+            // columns outside the target land in the line's guard and tail,
+            // which nothing reads.
+            if constexpr (STORE)
+                __builtin_amdgcn_raw_buffer_store_b32((uint32_t)h[R - 1], ls.rsrc, (int)voff, 4 * u, 0);
             b = bn;
             bn = bnn;
 #pragma unroll
@@ -308,13 +319,13 @@ __device__ __forceinline__ int run_strip(const dev::lds_u32* prof, const TargetS
 // First strips: R0 rows per lane, R0 in kRowStep..RMAX.
 template <int G, int RMAX, bool STORE, int R = kRowStep>
 __device__ __forceinline__ int run_first_strip(int r0, const dev::lds_u32* prof, const TargetState& t, int32_t* line,
-                               volatile dev::lds_u8* cring, int wchunks, int k)
+                               volatile dev::lds_u8* cring, const LineStore& ls, int wchunks, int k)
 {
     if constexpr (R >= RMAX) {
-        return run_strip<G, RMAX, false, STORE>(prof, t, line, cring, wchunks, k);
+        return run_strip<G, RMAX, false, STORE>(prof, t, line, cring, ls, wchunks, k);
     } else {
-        if (r0 == R) return run_strip<G, R, false, STORE>(prof, t, line, cring, wchunks, k);
-        return run_first_strip<G, RMAX, STORE, R + kRowStep>(r0, prof, t, line, cring, wchunks, k);
+        if (r0 == R) return run_strip<G, R, false, STORE>(prof, t, line, cring, ls, wchunks, k);
+        return run_first_strip<G, RMAX, STORE, R + kRowStep>(r0, prof, t, line, cring, ls, wchunks, k);
     }
 }
 
@@ -348,8 +359,18 @@ __device__ __forceinline__ void run_item(const BatchArgs& a, const int4 item, de
     constexpr int STRIP_MAX = G * RMAX;
     const int g = lane / G;
     const int k = lane % G;
-    int32_t* line = lines + (int64_t)g * a.line_stride;
+    int32_t* line = lines + (int64_t)g * a.line_stride + kLineGuard;
     volatile dev::lds_u8* cring = rings + g * 4 * G;
+    // The wave's line regions as one buffer (a wave-uniform descriptor).
+    LineStore ls;
+    {
+        const uint64_t wb = (uint64_t)(uintptr_t)lines;
+        const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)wb);
+        const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(wb >> 32));
+        ls.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)(((uint64_t)hi << 32) | lo), 0,
+                                                    kMaxGroups * a.line_stride * 4, kRawBufferFlags);
+        ls.lane_off = k == G - 1 ? (uint32_t)(g * a.line_stride + kLineGuard) * 4u : kOutOfRange;
+    }
     const SeqInfo qi = a.info[item.x];
     const int Lq = qi.z;
     const uint16_t* qseq = a.blocks + qi.x;
@@ -389,11 +410,11 @@ __device__ __forceinline__ void run_item(const BatchArgs& a, const int4 item, de
             }
             const bool store = st + 1 < nst;
             if (st == 0) {
-                res = store ? run_first_strip<G, RMAX, true>(R0, prof, t, line, cring, wchunks, k)
-                            : run_first_strip<G, RMAX, false>(R0, prof, t, line, cring, wchunks, k);
+                res = store ? run_first_strip<G, RMAX, true>(R0, prof, t, line, cring, ls, wchunks, k)
+                            : run_first_strip<G, RMAX, false>(R0, prof, t, line, cring, ls, wchunks, k);
             } else {
-                res = store ? run_strip<G, RMAX, true, true>(prof, t, line, cring, wchunks, k)
-                            : run_strip<G, RMAX, true, false>(prof, t, line, cring, wchunks, k);
+                res = store ? run_strip<G, RMAX, true, true>(prof, t, line, cring, ls, wchunks, k)
+                            : run_strip<G, RMAX, true, false>(prof, t, line, cring, ls, wchunks, k);
             }
             // The next strip reads this strip's bottom row from `line`.
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");

@@ -483,7 +483,7 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
             hp.batch_grid[c] = (int)std::min<int64_t>((int64_t)per_cu * dev.compute_units, (n + 3) / 4);
         }
         // Lines are touched only by queries of more than one strip.
-        hp.batch_stride = need_lines ? ((max_t + 63) & ~63) : 0;
+        hp.batch_stride = need_lines ? batched_line_stride(max_t) : 0;
         hp.info.bins = __builtin_popcount(used);
     }
 

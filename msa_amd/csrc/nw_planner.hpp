@@ -32,6 +32,13 @@ constexpr int kTileConfigCount = 3;
 constexpr TileConfig kTileConfig[kTileConfigCount] = {{16, 32}, {16, 16}, {32, 16}};
 constexpr int kMaxTileBytes = 512;     // largest lanes*rows: one code's profile block
 constexpr int kRowStep = 2;            // rows-per-lane granularity of a query's first strip
+// A batched lane group's strip-boundary line: kLineGuard ints in front of
+// column 0 and room for every column a strip's wavefront reaches (up to 2G
+// past the longest target), so that the bottom row is stored without bounds
+// checks; a region of batched_line_stride(longest target) ints per group.
+constexpr int kLineGuard = 64;
+constexpr int kLineMargin = 128;
+inline int32_t batched_line_stride(int64_t max_len) { return (int32_t)((max_len + kLineMargin + 63) & ~int64_t(63)); }
 int tile_config_index(int lanes, int rows);   // -1 when not in the table
 
 int batched_threads();
