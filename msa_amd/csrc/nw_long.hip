@@ -27,6 +27,7 @@
 // the neutral code, so a DNA strip profile is 5 x 64 x R bytes.
 #include "msa_internal.hpp"
 #include "nw_device.hpp"
+#include "nw_planner.hpp"
 
 #include <type_traits>
 
@@ -155,9 +156,20 @@ __device__ __forceinline__ bool wait_for(int* progress, int need, int32_t* error
 // a per-chunk base plus u -- one LDS byte read two steps ahead instead of two
 // DPP moves; each chunk's slots go in before step 61 of the chunk before.
 // The reads are volatile so that they stay single-byte reads.
+// The bottom-row store of a strip: the workgroup's boundary lines as one raw
+// buffer (wave-uniform descriptor; dword 3: 32-bit data format, as for any
+// raw buffer access on gfx9) and this lane's byte offset of the strip's line
+// -- out of range, so the store is dropped, on every lane but the last.
+constexpr int kRawBufferFlags = 0x00020000;
+constexpr uint32_t kOutOfRange = 0x80000000u;
+struct LineStore {
+    __amdgpu_buffer_rsrc_t rsrc;
+    uint32_t lane_off;
+};
+
 template <int R, bool LOAD, bool STORE>
 __device__ int run_strip(const dev::lds_u32* prof, const dev::lds_u8* remap, int null_slot, const Target& t,
-                         const int32_t* line_in, int* progress_in, int32_t* line_out, int* progress_out,
+                         const int32_t* line_in, int* progress_in, const LineStore& out, int* progress_out,
                          int32_t* error, volatile dev::lds_u8* ring, int k)
 {
     constexpr int W = R / 4;
@@ -184,7 +196,7 @@ __device__ int run_strip(const dev::lds_u32* prof, const dev::lds_u8* remap, int
         ln = line_value(t, line_in, 0, k);
         ln_next = line_value(t, line_in, 1, k);
     }
-    int outv = 0;
+    uint32_t voff = 0;
     // Slots of steps 0 and 1 (virtual columns -k and 1-k).
     const int base0 = (RING - k) & (RING - 1);
     int b = ring[base0];
@@ -203,7 +215,10 @@ __device__ int run_strip(const dev::lds_u32* prof, const dev::lds_u8* remap, int
         if constexpr (LOAD) ln = LAST ? ln_next : dev::dpp<dev::kWaveShl1>(ln, ln);
         dev::cells<R>(h, hup, dtop, w);
         dtop = hup;
-        if constexpr (STORE) outv = dev::dpp<dev::kWaveShl1>(h[R - 1], outv);
+        // The last lane's bottom cell goes straight to the line (column
+        // c*64 + u - 63 - lpad): one buffer store, the step's 4u in the
+        // scalar offset; columns left of the target land in the guard.
+        if constexpr (STORE) __builtin_amdgcn_raw_buffer_store_b32((uint32_t)h[R - 1], out.rsrc, (int)voff, 4 * u, 0);
         b = bn;
         bn = bnn;
 #pragma unroll
@@ -212,6 +227,7 @@ __device__ int run_strip(const dev::lds_u32* prof, const dev::lds_u8* remap, int
 
     for (int c = 0; c < t.chunks && ok; ++c) {
         const int base = (c * kLanes + 2 - k) & (RING - 1);
+        if constexpr (STORE) voff = out.lane_off + (uint32_t)(c * kLanes - (kLanes - 1) - t.lpad) * 4u;
 #pragma unroll 9
         for (int u = 0; u < kLanes - 3; ++u) step(std::false_type {}, base, u);
         // Chunk c+1 replaces chunk c-1, whose last reader was step 60.
@@ -224,9 +240,7 @@ __device__ int run_strip(const dev::lds_u32* prof, const dev::lds_u8* remap, int
         step(std::false_type {}, base, kLanes - 2);
         step(std::true_type {}, base, kLanes - 1);
         if constexpr (STORE) {
-            // Lane k now holds the bottom score of column c*64 + k - 63 - lpad.
-            const int col = c * kLanes + k - (kLanes - 1) - t.lpad;
-            if (col >= 0 && col < t.len) line_out[col] = outv;
+            // Columns up to c*64 - lpad are stored.
             const int done = min(t.len, c * kLanes + 1 - t.lpad);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             if (k == 0 && done > 0) __hip_atomic_store(progress_out, done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -270,6 +284,14 @@ __global__ __launch_bounds__(NW * 64) void nw_long_kernel(LongArgs a)
     }
     const int null_slot = a.slots - 1;
     int32_t* lines = a.lines + (int64_t)blockIdx.x * a.max_boundaries * a.line_stride;
+    __amdgpu_buffer_rsrc_t lines_rsrc;
+    {
+        const uint64_t wb = (uint64_t)(uintptr_t)lines;
+        const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)wb);
+        const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(wb >> 32));
+        lines_rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)(((uint64_t)hi << 32) | lo), 0,
+                                                       a.max_boundaries * a.line_stride * 4, kRawBufferFlags);
+    }
 
     while (true) {
         __syncthreads();
@@ -299,8 +321,10 @@ __global__ __launch_bounds__(NW * 64) void nw_long_kernel(LongArgs a)
             __builtin_amdgcn_s_waitcnt(0xC07F);
             __builtin_amdgcn_wave_barrier();
             const bool load = st > 0, store = st + 1 < nst;
-            const int32_t* lin = lines + (int64_t)(st - 1) * a.line_stride;
-            int32_t* lout = lines + (int64_t)st * a.line_stride;
+            const int32_t* lin = lines + (int64_t)(st - 1) * a.line_stride + kLongLineGuard;
+            LineStore lout;
+            lout.rsrc = lines_rsrc;
+            lout.lane_off = lane == kLanes - 1 ? (uint32_t)(st * a.line_stride + kLongLineGuard) * 4u : kOutOfRange;
             int res;
             if (!load && !store)
                 res = run_strip<R, false, false>(prof, remap, null_slot, t, lin, progress, lout, progress, a.error, ring, lane);

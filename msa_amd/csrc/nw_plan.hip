@@ -515,7 +515,7 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
         hp.long_rows = long_rows_for_slots(hp.long_slots);
         const int strip = long_strip_rows(hp.long_rows);
         hp.long_boundaries = std::max(1, (max_q + strip - 1) / strip - 1);
-        hp.long_stride = (max_t + 63) & ~63;
+        hp.long_stride = long_line_stride(max_t);
         hp.long_lds = long_lds_bytes(hp.long_rows, hp.long_slots, hp.long_boundaries);
         if (hp.long_lds > dev.lds_per_block)
             return fail(MSA_ERR_TOO_LARGE, "long pairs need %zu bytes of LDS per workgroup", hp.long_lds);

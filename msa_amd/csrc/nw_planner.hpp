@@ -39,6 +39,10 @@ constexpr int kRowStep = 2;            // rows-per-lane granularity of a query's
 constexpr int kLineGuard = 64;
 constexpr int kLineMargin = 128;
 inline int32_t batched_line_stride(int64_t max_len) { return (int32_t)((max_len + kLineMargin + 63) & ~int64_t(63)); }
+// The long kernel's strip boundaries: kLongLineGuard ints before column 0 (a
+// strip's last lane reaches 126 columns left of the target), none after it.
+constexpr int kLongLineGuard = 128;
+inline int32_t long_line_stride(int64_t max_len) { return (int32_t)((max_len + kLongLineGuard + 63) & ~int64_t(63)); }
 int tile_config_index(int lanes, int rows);   // -1 when not in the table
 
 int batched_threads();

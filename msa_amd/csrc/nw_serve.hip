@@ -306,7 +306,7 @@ int msa_srv_open(msa_server* srv, const msa_srv_config* cfg)
         n.batch_lines = (size_t)per_cu * dev.compute_units * msa::batched_groups_per_block() *
                         (size_t)msa::batched_line_stride((int64_t)L);
         n.plain_lines = (size_t)msa::plain_grid(int64_t(1) << 40, dev.compute_units) * stride;
-        n.long_lines = (size_t)4 * dev.compute_units * ((L + 511) / 512) * stride;
+        n.long_lines = (size_t)4 * dev.compute_units * ((L + 511) / 512) * (size_t)msa::long_line_stride((int64_t)L);
         s->slot_need = n;
     }
 
