@@ -207,6 +207,45 @@ __device__ __forceinline__ int line_value(const TargetState& t, const int32_t* _
     return (idx >= 0 && idx < t.len) ? line[idx] : 0;
 }
 
+// A lane's column of the target, chunk after chunk (idx = chunk*G + k -
+// lpad), with its block and bit shift kept up to date by additions instead
+// of a division by 3 per chunk: a chunk moves G columns, i.e. G/3 blocks and
+// G%3 slots of 5 bits.
+template <int G>
+struct ColumnCursor {
+    int idx;
+    int blk;
+    int shift;      // 1 + 5 * (idx mod 3)
+
+    __device__ __forceinline__ ColumnCursor(const TargetState& t, int chunk, int k)
+    {
+        idx = chunk * G + k - t.lpad;
+        const int q = (idx >= 0 ? idx : idx - 2) / 3;   // floor, also left of the target
+        blk = q;
+        shift = 1 + 5 * (idx - 3 * q);
+    }
+    __device__ __forceinline__ void advance()
+    {
+        idx += G;
+        blk += G / 3;
+        shift += 5 * (G % 3);
+        if (shift > 11) {
+            shift -= 15;
+            blk += 1;
+        }
+    }
+    // The residue code (the neutral code outside the target), as col_code.
+    __device__ __forceinline__ int code(const TargetState& t) const
+    {
+        return (idx >= 0 && idx < t.len) ? min((t.seq[blk] >> shift) & 0x1F, kNull) : kNull;
+    }
+    // The line value (0 outside the target), as line_value.
+    __device__ __forceinline__ int line(const TargetState& t, const int32_t* __restrict__ l) const
+    {
+        return (idx >= 0 && idx < t.len) ? l[idx] : 0;
+    }
+};
+
 // One strip of R rows per lane over the whole target.  LOAD: the strip above
 // left its bottom row in `line`; STORE: this strip leaves its bottom row there.
 // Returns (on the group's last lane) the last row's score at the last column.
@@ -249,6 +288,7 @@ __device__ __forceinline__ int run_strip(const dev::lds_u32* prof, const TargetS
         cring[RING + k] = c0;
     }
     int cc_next = col_code<G>(t, 1, k);
+    ColumnCursor<G> cur(t, 2, k);           // the column fetched at the end of chunk c is chunk c+2's
     int ln = 0, ln_next = 0;
     if constexpr (LOAD) {
         ln = line_value<G>(t, line, 0, k);
@@ -310,8 +350,9 @@ __device__ __forceinline__ int run_strip(const dev::lds_u32* prof, const TargetS
         });
         if constexpr (LOAD) ln = ln_next;
         result = (c == t.chunks - 1) ? h[R - 1] : result;
-        cc_next = col_code<G>(t, c + 2, k);
-        if constexpr (LOAD) ln_next = line_value<G>(t, line, c + 2, k);
+        cc_next = cur.code(t);
+        if constexpr (LOAD) ln_next = cur.line(t, line);
+        cur.advance();
     }
     return result;
 }
