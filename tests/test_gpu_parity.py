@@ -8,6 +8,8 @@ sample, and symmetry.  The bar is bit-exact int32 == the reference's float.
 """
 import ctypes
 import json
+import pathlib
+import re
 import zlib
 
 import numpy as np
@@ -319,16 +321,25 @@ def test_batched_tile_shapes(lanes, rows, lo, hi):
     assert_oracle(db, T, p, kernel="batched", lanes=lanes, rows=rows)
 
 
+def first_strip_row_step():
+    """kRowStep of msa_amd/csrc/nw_planner.hpp: the rows-per-lane granularity
+    of a query's first strip in the batched kernel."""
+    text = (pathlib.Path(__file__).resolve().parents[1] / "msa_amd" / "csrc" / "nw_planner.hpp").read_text()
+    return int(re.search(r"constexpr int kRowStep = (\d+);", text).group(1))
+
+
 @pytest.mark.parametrize("lanes,rows", TILES)
 def test_batched_every_first_strip_height(lanes, rows):
-    """Every first-strip height (2, 4, .., rows rows per lane) under every
-    tile shape, for queries of one strip and of two: query lengths are picked
-    in each height's range of residues, and every pair is scored."""
+    """Every first-strip height (kRowStep, 2 kRowStep, .., rows rows per lane,
+    the step read from the planner's header) under every tile shape, for
+    queries of one strip and of two: query lengths are picked in each
+    height's range of residues, and every pair is scored."""
     rng = np.random.default_rng(7 * lanes + rows)
     T, p = H.tables()["blosum62"]
-    step = 2 * lanes
+    row_step = first_strip_row_step()
+    step = row_step * lanes
     lengths = []
-    for j in range(1, rows // 2 + 1):
+    for j in range(1, rows // row_step + 1):
         rem = step * (j - 1) + 1 + int(rng.integers(0, step))
         lengths += [rem, rem + lanes * rows]
     codes = [rng.integers(0, 24, n).astype(np.uint8) for n in lengths]
