@@ -375,10 +375,17 @@ class Server:
 
     def fetch(self, job):
         """(int32 scores in pair order, stats dict) of a finished job."""
-        n = self.npairs.pop(job)
+        n = self.npairs.get(job)
+        if n is None:
+            raise MsaError(MSA_ERR_NOT_FOUND, "unknown job %d" % job)
         out = np.zeros(n, dtype=np.int32)
         st = SrvJobStats()
-        check(lib().msa_srv_fetch(self.handle, job, _ptr(out), n, ctypes.byref(st)))
+        rc = lib().msa_srv_fetch(self.handle, job, _ptr(out), n, ctypes.byref(st))
+        # The server forgets a job once it has handed it out (scores or its
+        # failure); a job not finished yet stays known on both sides.
+        if rc not in (MSA_ERR_NOT_READY, MSA_ERR_INVALID_ARGUMENT):
+            self.npairs.pop(job, None)
+        check(rc)
         return out, {f: getattr(st, f) for f, _ in SrvJobStats._fields_}
 
     def stats(self):

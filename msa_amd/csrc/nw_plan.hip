@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <atomic>
 #include <mutex>
@@ -651,9 +652,18 @@ struct msa_nw_plan_opaque {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int32_t* graph_out = nullptr;
+    // Every execution uses the plan's one scratch block (queue counters,
+    // strip-boundary lines, the error flag), so executions are serialized:
+    // each records `last_done` on its stream, and an execution on another
+    // stream first waits for it.
+    std::mutex lock;
+    hipEvent_t last_done = nullptr;
+    hipStream_t last_stream = nullptr;
+    bool has_last = false;
 
     ~msa_nw_plan_opaque()
     {
+        if (last_done) (void)hipEventDestroy(last_done);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (capture_stream) (void)hipStreamDestroy(capture_stream);
@@ -725,6 +735,28 @@ int msa_nw_plan_execute(msa_nw_plan plan, int32_t* d_out, msa_stream stream)
     if (!plan || (!d_out && plan->hp.info.pairs)) return fail(MSA_ERR_INVALID_ARGUMENT, "null plan or output");
     if (!plan->hp.info.pairs) return MSA_OK;
     const hipStream_t s = (hipStream_t)stream;
+    std::lock_guard<std::mutex> guard(plan->lock);
+    // Under a caller's capture the order is the caller's graph's business (a
+    // replay must not overlap another execution of this plan); otherwise wait
+    // for the previous execution when it ran on another stream.
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    MSA_HIP(hipStreamIsCapturing(s, &capture));
+    const bool eager = capture == hipStreamCaptureStatusNone;
+    if (eager && plan->has_last && plan->last_stream != s) MSA_HIP(hipStreamWaitEvent(s, plan->last_done, 0));
+    MSA_TRY(msa::execute_locked(plan, d_out, s));
+    if (eager) {
+        if (!plan->last_done) MSA_HIP(hipEventCreateWithFlags(&plan->last_done, hipEventDisableTiming));
+        MSA_HIP(hipEventRecord(plan->last_done, s));
+        plan->last_stream = s;
+        plan->has_last = true;
+    }
+    return MSA_OK;
+}
+
+}  // extern "C"
+
+int msa::execute_locked(msa_nw_plan plan, int32_t* d_out, hipStream_t s)
+{
     if (!plan->use_graph) return msa::launch_plan(plan->hp, plan->lay, plan->d_data, plan->scratch, d_out, s);
     if (plan->graph_out != d_out || !plan->exec) {
         if (plan->exec) { (void)hipGraphExecDestroy(plan->exec); plan->exec = nullptr; }
@@ -742,6 +774,8 @@ int msa_nw_plan_execute(msa_nw_plan plan, int32_t* d_out, msa_stream stream)
     }
     return msa::hip_status(hipGraphLaunch(plan->exec, s), "hipGraphLaunch");
 }
+
+extern "C" {
 
 int msa_nw_plan_info_get(msa_nw_plan plan, msa_nw_plan_info* info)
 {
@@ -796,6 +830,10 @@ int msa_nw_align(const uint16_t* blocks, const int64_t* offset, const int32_t* s
 
 namespace {
 
+// One workspace per (device, stream): calls on different streams never share
+// memory, calls on one stream run in stream order.  A workspace grows on
+// demand (hipFree waits for the device, so no earlier launch still reads the
+// old buffers) and lives until the process ends.
 struct PlainWorkspace {
     std::mutex lock;
     msa::SeqInfo* info = nullptr;
@@ -804,7 +842,16 @@ struct PlainWorkspace {
     size_t lines_cap = 0;
 };
 
-PlainWorkspace g_workspace[64];
+std::mutex g_workspaces_lock;
+std::map<std::pair<int, hipStream_t>, std::unique_ptr<PlainWorkspace>> g_workspaces;
+
+PlainWorkspace& workspace_for(int dev, hipStream_t stream)
+{
+    std::lock_guard<std::mutex> guard(g_workspaces_lock);
+    auto& slot = g_workspaces[{dev, stream}];
+    if (!slot) slot.reset(new PlainWorkspace());
+    return *slot;
+}
 
 template <typename T>
 int grow(T*& ptr, size_t& cap, size_t need)
@@ -839,13 +886,20 @@ extern "C" int msa_nw_plain(const uint16_t* d_blocks, const int64_t* d_offset, c
     if (dev < 0 || dev >= 64) return fail(MSA_ERR_INVALID_ARGUMENT, "device %d out of range", dev);
     msa::DeviceShape shape;
     MSA_TRY(msa::device_shape(shape));
-    PlainWorkspace& ws = g_workspace[dev];
+    const hipStream_t s = (hipStream_t)stream;
+    // A captured graph would keep the workspace's pointers, which a later,
+    // larger call frees.
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    MSA_HIP(hipStreamIsCapturing(s, &capture));
+    if (capture != hipStreamCaptureStatusNone)
+        return fail(MSA_ERR_INVALID_ARGUMENT, "msa_nw_plain allocates its workspace and cannot be captured; "
+                    "capture msa_nw_plan_execute instead");
+    PlainWorkspace& ws = workspace_for(dev, s);
     std::lock_guard<std::mutex> guard(ws.lock);
     const int grid = msa::plain_grid(npairs, shape.compute_units);
     const int32_t stride = (int32_t)(((int64_t)3 * max_blocks + 63) & ~int64_t(63));
     MSA_TRY(grow(ws.info, ws.info_cap, (size_t)nseq));
     MSA_TRY(grow(ws.lines, ws.lines_cap, (size_t)grid * std::max(stride, 1)));
-    const hipStream_t s = (hipStream_t)stream;
     MSA_TRY(msa::seq_info_launch(d_blocks, d_offset, d_size, nseq, ws.info, s));
     return msa::plain_launch(d_blocks, ws.info, d_pairs, nullptr, npairs, sxx, penalty, d_out, ws.lines,
                              std::max(stride, 1), grid, s);

@@ -134,6 +134,10 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
               const msa_nw_options& opt, const Tuning* tuning = nullptr);
 
 // msa_nw_plan_create with an explicit tuning table (null: from the options).
+// One execution of a plan on `stream` (graph or eager), with the plan's lock
+// held by the caller.
+int execute_locked(msa_nw_plan plan, int32_t* d_out, hipStream_t stream);
+
 int create_plan(msa_nw_plan* plan, const uint16_t* blocks, const int64_t* offset, const int32_t* size,
                 int32_t nseq, const int32_t* pairs, int64_t npairs, const int32_t* table625, int32_t penalty,
                 const msa_nw_options& opt, const Tuning* tuning, hipStream_t stream);

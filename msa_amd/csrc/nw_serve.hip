@@ -256,6 +256,11 @@ void msa_server_opaque::run()
             slot.job = nullptr;
             const int rc = admit(job, slot, si);
             if (rc != MSA_OK) {
+                // Whatever admit() queued before failing may still read or
+                // write the job's arena blocks: drain the slot's stream before
+                // another job can be given them.
+                (void)hipStreamSynchronize(slot.stream);
+                (void)hipGetLastError();
                 std::lock_guard<std::mutex> guard(mu);
                 job->status = rc;
                 job->error = msa_last_error();

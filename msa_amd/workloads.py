@@ -97,6 +97,17 @@ def unpack(blocks, offset, size):
     return np.stack([(b >> 1) & 31, (b >> 6) & 31, (b >> 11) & 31], axis=1).reshape(-1).astype(np.uint8)
 
 
+def pair_of(c):
+    """(i, j) of pair c in the all-pairs order i = 1..n-1, j < i, c = i(i-1)/2 + j
+    (src/pairwise/pairwise.cpp:79-81)."""
+    i = int((1 + np.sqrt(1 + 8 * float(c))) // 2)
+    while i * (i - 1) // 2 > c:
+        i -= 1
+    while (i + 1) * i // 2 <= c:
+        i += 1
+    return i, c - i * (i - 1) // 2
+
+
 class Workload:
     def __init__(self, name, codes, table, penalty, description):
         self.name = name

@@ -238,7 +238,7 @@ class ReferenceBuild:
     def __init__(self):
         self.lib = ctypes.CDLL(self.PATH)
         self.lib.ref_align_list.argtypes = [
-            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
             ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p, ctypes.c_int]
         self.lib.ref_align_list.restype = ctypes.c_int
         self.lib.ref_align_pair.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
@@ -254,6 +254,9 @@ class ReferenceBuild:
         return [int(v) for v in out[:n]]
 
     def align_list(self, blocks, offsets, nblocks, pairs, table625, penalty, workers=1):
+        """Scores of an explicit pair list through the reference's own align()
+        (sequential.cpp:77-95) over a museqa::database of every sequence, which
+        chooses each pair's orientation itself."""
         blocks = np.ascontiguousarray(blocks, dtype=np.uint16)
         offsets = np.ascontiguousarray(offsets, dtype=np.int64)
         nblocks = np.ascontiguousarray(nblocks, dtype=np.int64)
@@ -261,9 +264,11 @@ class ReferenceBuild:
         table = np.ascontiguousarray(np.asarray(table625, dtype=np.float32).reshape(625))
         out = np.zeros(len(pairs), dtype=np.float32)
         if len(pairs):
-            self.lib.ref_align_list(blocks.ctypes.data if len(blocks) else None, offsets.ctypes.data,
-                                    nblocks.ctypes.data, pairs.ctypes.data, len(pairs), table.ctypes.data,
-                                    float(penalty), out.ctypes.data, int(workers))
+            rc = self.lib.ref_align_list(blocks.ctypes.data if len(blocks) else None, offsets.ctypes.data,
+                                         nblocks.ctypes.data, len(nblocks), pairs.ctypes.data, len(pairs),
+                                         table.ctypes.data, float(penalty), out.ctypes.data, int(workers))
+            if rc != 0:
+                raise ValueError("pair list names a sequence outside the database")
         return out
 
 

@@ -276,9 +276,31 @@ def test_full_uniform_batch_512():
     _full_size_check(W.uniform_batch(H.tables()))
 
 
-def test_full_uneven_batch_2000():
-    """2000 sequences, log-normal lengths: all 1,999,000 pairs, batched == plain."""
-    _full_size_check(W.uneven_batch(H.tables()), sample=300)
+def test_full_uneven_batch_2000_graph_replays():
+    """2000 sequences, log-normal lengths: all 1,999,000 pairs scored by the
+    hipGraph-captured plan the bench times (the first execution captures and
+    launches, the next ones replay the instantiated graph), each replay equal
+    to the plain kernel on every pair, and a random sample equal to the oracle."""
+    w = W.uneven_batch(H.tables())
+    db = (w.blocks, w.offsets, w.sizes)
+    plain = gpu_scores(db, w.table, w.penalty, kernel="plain")
+    plan = N.Plan(*db, w.table, w.penalty, opts=N.options(N.KERNEL_AUTO, use_graph=True))
+    out = N.DeviceBuffer(4 * w.npairs)
+    try:
+        for replay in range(3):
+            N.check(N.lib().msa_memset(out.ptr, 0xAB, 4 * w.npairs))
+            plan.execute(out.ptr)
+            plan.status()
+            got = out.download(np.int32, w.npairs)
+            assert np.array_equal(got, plain), "replay %d differs from the plain kernel" % replay
+    finally:
+        plan.close()
+        out.close()
+    rng = np.random.default_rng(0)
+    idx = rng.choice(w.npairs, 300, replace=False)
+    pairs = np.array([divmod_pair(int(c)) for c in idx], dtype=np.int32)
+    exp = H.oracle_scores(db, w.table, w.penalty, pairs=pairs, workers=16)
+    assert np.array_equal(plain[idx].astype(np.float32), exp)
 
 
 def test_long_dna_sample():
@@ -287,6 +309,107 @@ def test_long_dna_sample():
     rng = np.random.default_rng(1)
     pairs = rng.choice(64, (12, 2)).astype(np.int32)
     assert_oracle((w.blocks, w.offsets, w.sizes), w.table, w.penalty, pairs=pairs, kernel="auto")
+
+
+def test_long_dna_full_scale():
+    """All 2,016 pairs of the 64 x 30,000 DNA set through the long kernel (the
+    auto plan routes every pair there): many rounds of the persistent queue and
+    its last, partial round.  Every pair equals the plain kernel, 8 pairs equal
+    the oracle, and the kernel's bounded wait never gave up."""
+    w = W.long_dna()
+    db = (w.blocks, w.offsets, w.sizes)
+    plan = N.Plan(*db, w.table, w.penalty, opts=N.options(N.KERNEL_AUTO, use_graph=True))
+    out = N.DeviceBuffer(4 * w.npairs)
+    try:
+        info = plan.info()
+        assert info.long_pairs == w.npairs == 2016
+        plan.execute(out.ptr)
+        plan.status()
+        got = out.download(np.int32, w.npairs)
+    finally:
+        plan.close()
+        out.close()
+    plain = gpu_scores(db, w.table, w.penalty, kernel="plain")
+    assert np.array_equal(got, plain)
+    idx = np.random.default_rng(2).choice(w.npairs, 8, replace=False)
+    pairs = np.array([divmod_pair(int(c)) for c in idx], dtype=np.int32)
+    exp = H.oracle_scores(db, w.table, w.penalty, pairs=pairs, workers=8)
+    assert np.array_equal(got[idx].astype(np.float32), exp)
+
+
+def test_plan_executions_on_two_streams_back_to_back():
+    """One plan (plain, batched and long pairs; eager and graph forms)
+    executed on two streams with no synchronisation between: the plan
+    serializes its executions (they share its scratch), so both outputs equal
+    an eager run every time."""
+    lib = N.lib()
+    rng = np.random.default_rng(31)
+    T, p = H.tables()["blosum62"]
+    db = H.random_db(rng, 36, 5, 1500, dirty=True)
+    n = 36 * 35 // 2
+    ref = gpu_scores(db, T, p, long_threshold=1000)
+    assert np.array_equal(ref.astype(np.float32), H.oracle_scores(db, T, p))
+    streams = [ctypes.c_void_p(), ctypes.c_void_p()]
+    for s in streams:
+        N.check(lib.msa_stream_create(ctypes.byref(s)))
+    outs = [N.DeviceBuffer(4 * n), N.DeviceBuffer(4 * n)]
+    try:
+        for use_graph in (False, True):
+            plan = N.Plan(*db, T, p, opts=N.options(N.KERNEL_AUTO, use_graph=use_graph, long_threshold=1000))
+            info = plan.info()
+            assert info.plain_pairs > 0 and info.batched_pairs > 0 and info.long_pairs > 0
+            for _ in range(3):
+                for o in outs:
+                    N.check(lib.msa_memset(o.ptr, 0xAB, 4 * n))
+                plan.execute(outs[0].ptr, streams[0].value)
+                plan.execute(outs[1].ptr, streams[1].value)
+                plan.execute(outs[0].ptr, streams[0].value)
+                for s in streams:
+                    N.check(lib.msa_stream_sync(s))
+                plan.status(streams[0].value)
+                for o in outs:
+                    assert np.array_equal(o.download(np.int32, n), ref)
+            plan.close()
+    finally:
+        for o in outs:
+            o.close()
+        for s in streams:
+            N.check(lib.msa_stream_destroy(s))
+
+
+def test_plain_entry_refuses_stream_capture():
+    """msa_nw_plain grows and frees a workspace of its own, so it refuses to be
+    captured into a caller's graph (MSA_ERR_INVALID_ARGUMENT) and leaves the
+    capture usable."""
+    lib = N.lib()
+    rng = np.random.default_rng(32)
+    T, p = H.tables()["blosum62"]
+    blocks, offsets, sizes = H.random_db(rng, 6, 1, 100)
+    d_blocks, d_off, d_size = N.DeviceBuffer(blocks.nbytes), N.DeviceBuffer(offsets.nbytes), N.DeviceBuffer(sizes.nbytes)
+    d_blocks.upload(blocks)
+    d_off.upload(offsets)
+    d_size.upload(sizes)
+    d_out = N.DeviceBuffer(4 * 15)
+    table = np.ascontiguousarray(T, dtype=np.int32)
+    stream = ctypes.c_void_p()
+    N.check(lib.msa_stream_create(ctypes.byref(stream)))
+    plan = N.Plan(blocks, offsets, sizes, T, p)
+    graph = ctypes.c_void_p()
+    try:
+        N.check(lib.msa_graph_capture_begin(stream))
+        rc = lib.msa_nw_plain(d_blocks.ptr, d_off.ptr, d_size.ptr, 6, int(sizes.max()), None, 15,
+                              table.ctypes.data, p, d_out.ptr, stream)
+        assert rc == N.MSA_ERR_INVALID_ARGUMENT
+        plan.execute(d_out.ptr, stream.value)
+        N.check(lib.msa_graph_capture_end(stream, ctypes.byref(graph)))
+        N.check(lib.msa_graph_launch(graph, stream))
+        N.check(lib.msa_stream_sync(stream))
+        exp = H.oracle_scores((blocks, offsets, sizes), T, p)
+        assert np.array_equal(d_out.download(np.int32, 15).astype(np.float32), exp)
+    finally:
+        N.check(lib.msa_graph_destroy(graph))
+        plan.close()
+        N.check(lib.msa_stream_destroy(stream))
 
 
 def test_gap_code_rejected_on_device_path_too():

@@ -18,6 +18,7 @@ import numpy as np
 import pytest
 
 from oracle import museqa_ref as M
+from msa_amd import workloads as W
 from tests import helpers as H
 
 REF_TABLE_SRC = "/root/reference/src/pairwise/table.cu"
@@ -132,6 +133,46 @@ def test_oracle_matches_python_restatement_on_padding_codes():
         if ref is not None:
             pairs = np.array(M.pair_order(4), dtype=np.int32)
             assert np.array_equal(got, ref.align_list(db[0], db[1], db[2], pairs, T, p))
+
+
+@pytest.mark.skipif(not M.ReferenceBuild.available(), reason="reference build (oracle/_ref) not present")
+def test_oracle_orientation_matches_reference_align():
+    """The orientation rule -- rows = the sequence with more blocks, on a tie
+    pair.second (sequential.cpp:87-91) -- pinned to the reference's own
+    align() (oracle/_ref, the TU's file-local align over a museqa::database),
+    not to a restatement: 4 databases x 4 asymmetric tables x 66 pairs with
+    many equal block counts, internal padding codes and explicit lists that
+    hold both orders of a pair.  The same lists with every pair reversed must
+    give different scores somewhere, so the check would catch a flipped tie."""
+    ref = M.ReferenceBuild()
+    rng = np.random.default_rng(41)
+    T, p = H.tables()["blosum62"]
+    cases = sensitive = 0
+    for trial in range(4):
+        # Lengths 3k..3k+2 for a few k: many pairs share a block count.
+        seqs = []
+        for _ in range(12):
+            k = int(rng.integers(1, 5)) * 4
+            c = rng.integers(0, 24, 3 * k + int(rng.integers(0, 3))).astype(np.uint8)
+            if rng.random() < 0.4:
+                c[int(rng.integers(0, len(c)))] = 24
+            seqs.append(c)
+        db = W.pack(seqs)
+        pairs = np.array([(i, j) if rng.random() < 0.5 else (j, i) for i in range(12) for j in range(i)],
+                         dtype=np.int32)
+        for _ in range(4):
+            Ta = T.copy()
+            for _ in range(12):
+                a, b = rng.integers(0, 24, 2)
+                Ta[a, b] = int(rng.integers(-4, 12))
+            got = H.oracle_scores(db, Ta, p, pairs=pairs, workers=1)
+            exp = ref.align_list(db[0], db[1], db[2], pairs, Ta, p, workers=3)
+            assert np.array_equal(got, exp)
+            rev = H.oracle_scores(db, Ta, p, pairs=pairs[:, ::-1].copy(), workers=1)
+            sensitive += int(np.sum(rev != got))
+            cases += len(pairs)
+    assert cases >= 200
+    assert sensitive > 0
 
 
 def test_closed_forms():

@@ -111,3 +111,64 @@ def test_poisson_stream_with_concurrent_submitter():
             assert np.array_equal(scores, alone[k])
     finally:
         srv.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rate", [10.0, 500.0])
+def test_serving_workload_poisson(rate):
+    """The serving workload itself: jobs of 8-256 protein sequences (count
+    uniform), L ~ U[100, 2000], BLOSUM62 background residues, blosum62, seed
+    5; Poisson arrivals at `rate` jobs/s (seed 50), 24 jobs submitted from
+    another thread while results are collected.  Every job equals the same
+    job scored alone (msa_nw_align), a sample of its pairs equals the
+    oracle, and at 500 jobs/s jobs run on the device at the same time."""
+    tables = H.tables()
+    jobs = W.serving_jobs(tables, 24, seed=5)
+    assert min(j.nseq for j in jobs) >= 8 and max(j.nseq for j in jobs) <= 256
+    assert min(j.lengths.min() for j in jobs) >= 100 and max(j.lengths.max() for j in jobs) <= 2000
+    T, p = tables["blosum62"]
+    alone = [N.align(j.blocks, j.offsets, j.sizes, T, p) for j in jobs]
+    rng = np.random.default_rng(7)
+    for j, ref in zip(jobs, alone):
+        idx = rng.choice(j.npairs, 3, replace=False)
+        pairs = np.array([W.pair_of(int(c)) for c in idx], dtype=np.int32)
+        exp = H.oracle_scores((j.blocks, j.offsets, j.sizes), T, p, pairs=pairs)
+        assert np.array_equal(ref[idx].astype(np.float32), exp)
+    arrivals = W.poisson_arrivals(rate, len(jobs), seed=50)
+    srv = N.Server(T, p, slots=4, max_length=2048, device_bytes=4 << 30, host_bytes=512 << 20)
+    ids = [None] * len(jobs)
+    errors = []
+    try:
+        t0 = time.time()
+
+        def submitter():
+            try:
+                for k, j in enumerate(jobs):
+                    delay = arrivals[k] - (time.time() - t0)
+                    if delay > 0:
+                        time.sleep(delay)
+                    ids[k] = srv.submit(j.blocks, j.offsets, j.sizes)
+            except Exception as e:      # surfaced below
+                errors.append(e)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        stats = []
+        for k in range(len(jobs)):
+            while ids[k] is None and th.is_alive():
+                time.sleep(0.0005)
+            assert ids[k] is not None, errors
+            srv.wait(ids[k], 60000)
+            scores, st = srv.fetch(ids[k])
+            assert np.array_equal(scores, alone[k]), "job %d differs from the job scored alone" % k
+            assert st["pairs"] == jobs[k].npairs
+            stats.append(st)
+        th.join()
+        assert not errors
+        spans = sorted((st["admitted_us"], st["finished_us"]) for st in stats)
+        overlaps = sum(1 for a, b in zip(spans, spans[1:]) if b[0] < a[1])
+        if rate >= 500:
+            assert overlaps >= 1
+        assert srv.stats()["completed"] == len(jobs)
+    finally:
+        srv.close()
