@@ -229,6 +229,24 @@ int msa_nw_plain(const uint16_t* d_blocks, const int64_t* d_offset, const int32_
                  const int32_t* table625, int32_t penalty,
                  int32_t* d_out, msa_stream stream);
 
+/*
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): msa_nw_plain with the same arguments, scores and kernels,
+ * but its workspace (a SeqInfo per sequence, a strip-boundary line per
+ * workgroup) lives in a fixed 64 MiB device array that every GPU the library
+ * runs on holds until the process ends; it calls no hipMalloc or hipFree.
+ * Limits: at most 131,072 sequences, and max_blocks * 3 residues at most
+ * about 16.3 million (MSA_ERR_TOO_LARGE beyond); long sequences run on fewer
+ * workgroups so that their lines fit.  Every call on a device shares that
+ * device's array: calls that overlap on one device (other streams, other
+ * threads) race and must be serialized by the caller.
+ */
+int msa_nw_plain_noalloc(const uint16_t* d_blocks, const int64_t* d_offset, const int32_t* d_size, int32_t nseq,
+                         int32_t max_blocks, const int32_t* d_pairs, int64_t npairs,
+                         const int32_t* table625, int32_t penalty,
+                         int32_t* d_out, msa_stream stream);
+
 /* ---- phylogeny: neighbor joining (phylogeny::run, src/phylogeny/phylogeny.cuh:
  *      110-124; njoining::sequential_{symmetric,linear},
  *      src/phylogeny/njoining/impl/sequential.cpp:270-368) ---------------------

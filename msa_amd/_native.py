@@ -127,6 +127,7 @@ SIGNATURES = {
     "msa_nw_plan_status": (ctypes.c_int, [_vp, _vp]),
     "msa_nw_align": (ctypes.c_int, [_vp, _vp, _vp, _i32, _vp, _i64, _vp, _i32, ctypes.POINTER(NwOptions), _vp]),
     "msa_nw_plain": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _vp, _i64, _vp, _i32, _vp, _vp]),
+    "msa_nw_plain_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _vp, _i64, _vp, _i32, _vp, _vp]),
     "msa_nw_tune": (ctypes.c_int, [_vp, _vp, _vp, _i32, _vp, _i32, _i64, ctypes.c_char_p, _vp, ctypes.c_size_t]),
     "msa_nj_build": (ctypes.c_int, [_vp, _i32, _i32, _vp]),
     "msa_nj_build_device": (ctypes.c_int, [_vp, _i32, _i32, _vp, _vp]),
