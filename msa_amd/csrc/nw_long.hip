@@ -1,23 +1,50 @@
-// The long-sequence Needleman-Wunsch kernel: one pair per workgroup, its rows
-// cut into strips that the workgroup's waves score as a pipelined
-// anti-diagonal wavefront of tiles.
+// The long-sequence Needleman-Wunsch kernel: every pair's rows cut into
+// strips, and the strips of all pairs scored by one-wave workgroups that take
+// them from a global queue, each strip an anti-diagonal wavefront that
+// streams the bottom row of the strip above it from global memory.
 //
 // A pair whose shorter side runs to tens of thousands of residues is too big
-// for one lane group (the batched kernel) and too few pairs exist to fill the
-// GPU one pair per group.  Here a workgroup of NW waves takes one pair at a
-// time.  The query is cut into strips of 64*R rows (a first, partial strip at
-// the top takes the neutral padding rows); strip s goes to wave s % NW.  Each
-// wave runs its strip as a 64-lane wavefront (lane k owns R rows, bottom
+// for one lane group (the batched kernel), and too few such pairs exist to
+// fill the GPU one pair per group.  Here the query of each pair is cut into
+// strips of 64*R rows (a first, partial strip at the top takes the neutral
+// padding rows) and the work list holds one task per (pair, strip): the pairs
+// (costliest first) in groups as large as the grid, each group level by
+// level -- strip 0 of every pair of the group, then strip 1, ...  So the
+// strip above a task was handed out about a group earlier and is well on its
+// way when the task starts (the planner keeps about one resident wave per
+// long pair for this, two to four per SIMD, as many on every SIMD).
+// Persistent one-wave workgroups take tasks in
+// list order from one counter: a pair's strips go to whichever waves are free,
+// on any compute unit, and the last pairs spread over the whole GPU instead
+// of leaving it idle at the end.
+//
+// A wave runs its strip as a 64-lane wavefront (lane k owns R rows, bottom
 // scores move one lane per step by DPP wave shifts, each lane reads its
-// column's slot from a ring in LDS) over 64-column chunks, i.e. over a row of
-// 64R x 64 tiles.  The strip below
-// follows a few tiles behind: after every tile the upper wave stores the
-// strip's bottom row for those 64 columns to a line in global memory (the
-// int32 tile boundary) and publishes its progress in an LDS counter; the
-// lower wave waits for that counter before loading the boundary.  All waves
-// of a workgroup share one compute unit, so workgroup-scope release/acquire
-// orders those stores and loads.  Waiting is bounded: a wave that waits too
-// long raises the error flag and stops.
+// column's slot from a ring in LDS) over 64-column chunks.  The strip's
+// bottom cells (the last lane's, one per step) gather in a register that
+// moves down one lane per step, and every chunk ends with one coalesced store
+// of those 64 cells into a line in global memory; the strip below reads the
+// line two chunks ahead.  Every line word is 64
+// bits, (tag << 32) | score, stored with the cache policy of an agent-scope
+// atomic store (sc1) and loaded by relaxed agent-scope atomic loads, so the
+// words carry their own synchronisation: a reader takes a word only when its
+// tag is the producer's, and otherwise loads it again (the atomics of one
+// location are coherent, so the producer's word arrives).  No fence is needed
+// on the way, and a reader that keeps up with its producer just waits for it.
+//
+// Lines live in a ring of `nlines` slots; the strips that have a strip below
+// are numbered in list order, and strip o writes slot o % nlines as its
+// (o / nlines)-th occupant.  An occupant's tag is unique to it and to this
+// execution (a per-execution base from a counter that only grows, plus the
+// occupant number), so no earlier line is ever mistaken for it.  Before an
+// occupant writes, it waits (agent-scope acquire) until the slot's previous
+// line has been read in full: its reader advances the slot's turn with an
+// agent-scope release when its strip is done.  Every wait -- for a word, for
+// a turn -- is for a task handed out earlier, whose wave is running, so no
+// wait deadlocks whatever the grid (the planner makes the ring longer than
+// the distance between a strip and the strip below it in the list).  Waiting
+// is bounded: a wave that waits too long, or sees the error flag, raises the
+// flag and stops.
 //
 // Only pairs the batched kernel could take come here (no padding code
 // before the last residue of the reference's column sequence); the planner
@@ -29,6 +56,7 @@
 #include "nw_device.hpp"
 #include "nw_planner.hpp"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace msa {
@@ -36,20 +64,25 @@ namespace msa {
 namespace {
 
 constexpr int kLanes = 64;
-constexpr int kRingBytes = 4 * kLanes;   // one wave's slot ring
+constexpr int kRingBytes = 4 * kLanes;   // the wave's slot ring
 constexpr int kSpinLimit = 1 << 24;     // ~1e9 cycles of waiting before giving up
+constexpr int kSc1 = 16;                // buffer-instruction cache policy: sc1 (agent scope)
 
 struct LongArgs {
     const uint16_t* blocks;
     const SeqInfo* info;
     const int4* pairs;          // {query, target, output index, transpose flag}
-    int32_t npairs;
+    const int4* tasks;          // {pair, strip, line read (-1: none), line written (-1: none)}
+    int32_t ntasks;
     int32_t* queue;
     int32_t* error;
     int32_t* out;
-    int32_t* lines;             // per workgroup: max_boundaries lines of line_stride ints
+    uint64_t* lines;            // nlines lines of line_stride words: (tag << 32) | score
+    int32_t* turn;              // per line slot: occupants whose line has been read
+    const uint32_t* tag_word;   // this execution's tags end just below it
     int32_t line_stride;
-    int32_t max_boundaries;
+    int32_t nlines;
+    int32_t occupants;          // occupants per slot in one execution (at most)
     int32_t slots;              // profile slots, neutral slot last
     int32_t penalty;
     Sxx sxx;
@@ -129,24 +162,63 @@ __device__ __forceinline__ int col_slot(const Target& t, const dev::lds_u8* rema
     return (idx >= 0 && idx < t.len) ? (int)remap[dev::code_at(t.seq, idx)] : null_slot;
 }
 
-__device__ __forceinline__ int line_value(const Target& t, const int32_t* line, int chunk, int k)
+__device__ __forceinline__ void raise_error(int32_t* error)
 {
-    const int idx = chunk * kLanes + k - t.lpad;
-    return (idx >= 0 && idx < t.len) ? line[idx] : 0;
+    __hip_atomic_store(error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Waits until the strip above has published at least `need` columns.
-__device__ __forceinline__ bool wait_for(int* progress, int need, int32_t* error)
+__device__ __forceinline__ bool give_up(int spins, int32_t* error)
+{
+    if ((spins & 255) != 0) return false;
+    if (spins > kSpinLimit || __hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        raise_error(error);
+        return true;
+    }
+    return false;
+}
+
+// Waits until the slot's turn reaches `want`, then acquires (agent scope);
+// false when the wait gave up or the error flag is up.
+__device__ __forceinline__ bool wait_turn(const int32_t* turn, int want, int32_t* error)
 {
     int spins = 0;
-    while (__hip_atomic_load(progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < need) {
+    while (__hip_atomic_load(turn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < want) {
         __builtin_amdgcn_s_sleep(2);
-        if (++spins > kSpinLimit) {
-            if (__lane_id() == 0) atomicExch(error, 1);
-            return false;
+        if (give_up(++spins, error)) return false;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return true;
+}
+
+// Lane k's word of the line above for chunk `chunk` (0 outside the target).
+__device__ __forceinline__ uint64_t line_word(const Target& t, const uint64_t* line, int chunk, int k)
+{
+    const int idx = chunk * kLanes + k - t.lpad;
+    return (idx >= 0 && idx < t.len) ? __hip_atomic_load(line + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                     : 0;
+}
+
+// The score in a word loaded by line_word, once its tag is the producer's:
+// lanes holding an older word load it again until the producer's arrives.
+__device__ __forceinline__ bool take_word(const Target& t, const uint64_t* line, int chunk, int k, uint64_t raw,
+                                          uint32_t tag, int32_t* error, int& value)
+{
+    const int idx = chunk * kLanes + k - t.lpad;
+    const bool inside = idx >= 0 && idx < t.len;
+    bool stale = inside && (uint32_t)(raw >> 32) != tag;
+    if (__any(stale)) {
+        int spins = 0;
+        while (true) {
+            __builtin_amdgcn_s_sleep(1);
+            if (stale) {
+                raw = __hip_atomic_load(line + idx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                stale = (uint32_t)(raw >> 32) != tag;
+            }
+            if (!__any(stale)) break;
+            if (give_up(++spins, error)) return false;
         }
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    value = inside ? (int)(uint32_t)raw : 0;
     return true;
 }
 
@@ -156,21 +228,25 @@ __device__ __forceinline__ bool wait_for(int* progress, int need, int32_t* error
 // a per-chunk base plus u -- one LDS byte read two steps ahead instead of two
 // DPP moves; each chunk's slots go in before step 61 of the chunk before.
 // The reads are volatile so that they stay single-byte reads.
-// The bottom-row store of a strip: the workgroup's boundary lines as one raw
-// buffer (wave-uniform descriptor; dword 3: 32-bit data format, as for any
-// raw buffer access on gfx9) and this lane's byte offset of the strip's line
-// -- out of range, so the store is dropped, on every lane but the last.
+// The bottom-row store of a strip: the strip's line as one raw buffer
+// (wave-uniform descriptor; dword 3: 32-bit data format, as for any raw
+// buffer access on gfx9), one 8-byte word per lane and chunk.  One descriptor
+// spans one line, so every offset fits in 32 bits.
 constexpr int kRawBufferFlags = 0x00020000;
-constexpr uint32_t kOutOfRange = 0x80000000u;
-struct LineStore {
-    __amdgpu_buffer_rsrc_t rsrc;
-    uint32_t lane_off;
+typedef int v2i32 __attribute__((ext_vector_type(2)));
+
+// Where a strip reads the line above and writes its own.
+struct Exchange {
+    const uint64_t* line_in;    // column 0 of the line above
+    uint32_t tag_in;            // its producer's tag
+    __amdgpu_buffer_rsrc_t out; // this strip's line
+    uint32_t lane_off;          // byte offset of this lane's word of chunk 0 (column lane - 63 - lpad)
+    int32_t tag_out;
 };
 
 template <int R, bool LOAD, bool STORE>
 __device__ int run_strip(const dev::lds_u32* prof, const dev::lds_u8* remap, int null_slot, const Target& t,
-                         const int32_t* line_in, int* progress_in, const LineStore& out, int* progress_out,
-                         int32_t* error, volatile dev::lds_u8* ring, int k)
+                         const Exchange& x, int32_t* error, volatile dev::lds_u8* ring, int k, bool& ok)
 {
     constexpr int W = R / 4;
     constexpr int SLOT_WORDS = kLanes * W;
@@ -179,9 +255,7 @@ __device__ int run_strip(const dev::lds_u32* prof, const dev::lds_u8* remap, int
 #pragma unroll
     for (int r = 0; r < R; ++r) h[r] = 0;
     int dtop = 0;
-    bool ok = true;
 
-    if constexpr (LOAD) ok = wait_for(progress_in, min(t.len, 2 * kLanes - t.lpad), error);
     // Columns -64..-1 are neutral; chunk 0 goes in before the first read.
     ring[kLanes + k] = null_slot;
     ring[3 * kLanes + k] = null_slot;
@@ -192,11 +266,13 @@ __device__ int run_strip(const dev::lds_u32* prof, const dev::lds_u8* remap, int
     }
     int cc_next = col_slot(t, remap, null_slot, 1, k);
     int ln = 0, ln_next = 0;
+    uint64_t raw_next = 0;
     if constexpr (LOAD) {
-        ln = line_value(t, line_in, 0, k);
-        ln_next = line_value(t, line_in, 1, k);
+        const uint64_t raw0 = line_word(t, x.line_in, 0, k);
+        raw_next = line_word(t, x.line_in, 1, k);
+        ok = take_word(t, x.line_in, 0, k, raw0, x.tag_in, error, ln);
     }
-    uint32_t voff = 0;
+    int row = 0;
     // Slots of steps 0 and 1 (virtual columns -k and 1-k).
     const int base0 = (RING - k) & (RING - 1);
     int b = ring[base0];
@@ -215,10 +291,10 @@ __device__ int run_strip(const dev::lds_u32* prof, const dev::lds_u8* remap, int
         if constexpr (LOAD) ln = LAST ? ln_next : dev::dpp<dev::kWaveShl1>(ln, ln);
         dev::cells<R>(h, hup, dtop, w);
         dtop = hup;
-        // The last lane's bottom cell goes straight to the line (column
-        // c*64 + u - 63 - lpad): one buffer store, the step's 4u in the
-        // scalar offset; columns left of the target land in the guard.
-        if constexpr (STORE) __builtin_amdgcn_raw_buffer_store_b32((uint32_t)h[R - 1], out.rsrc, (int)voff, 4 * u, 0);
+        // The last lane's bottom cell enters the chunk's row collector, which
+        // moves down one lane per step: after step 63, lane u holds step u's
+        // cell (column c*64 + u - 63 - lpad).
+        if constexpr (STORE) row = dev::dpp<dev::kWaveShl1>(h[R - 1], row);
         b = bn;
         bn = bnn;
 #pragma unroll
@@ -227,7 +303,6 @@ __device__ int run_strip(const dev::lds_u32* prof, const dev::lds_u8* remap, int
 
     for (int c = 0; c < t.chunks && ok; ++c) {
         const int base = (c * kLanes + 2 - k) & (RING - 1);
-        if constexpr (STORE) voff = out.lane_off + (uint32_t)(c * kLanes - (kLanes - 1) - t.lpad) * 4u;
 #pragma unroll 9
         for (int u = 0; u < kLanes - 3; ++u) step(std::false_type {}, base, u);
         // Chunk c+1 replaces chunk c-1, whose last reader was step 60.
@@ -238,143 +313,156 @@ __device__ int run_strip(const dev::lds_u32* prof, const dev::lds_u8* remap, int
         }
         step(std::false_type {}, base, kLanes - 3);
         step(std::false_type {}, base, kLanes - 2);
+        // The line above for chunk c+1, loaded a chunk ago, is taken before
+        // the step that moves to it.
+        if constexpr (LOAD) ok = take_word(t, x.line_in, c + 1, k, raw_next, x.tag_in, error, ln_next);
         step(std::true_type {}, base, kLanes - 1);
+        // The chunk's bottom row and the tag: one 512-byte store of 8-byte
+        // words with the sc1 policy; columns left of the target land in the
+        // guard.
         if constexpr (STORE) {
-            // Columns up to c*64 - lpad are stored.
-            const int done = min(t.len, c * kLanes + 1 - t.lpad);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            if (k == 0 && done > 0) __hip_atomic_store(progress_out, done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const v2i32 word = {row, x.tag_out};
+            __builtin_amdgcn_raw_buffer_store_b64(word, x.out, (int)(x.lane_off + (uint32_t)(c * kLanes) * 8u), 0,
+                                                  kSc1);
         }
         cc_next = col_slot(t, remap, null_slot, c + 2, k);
-        if constexpr (LOAD) {
-            const int need = min(t.len, (c + 3) * kLanes - t.lpad);
-            if (need > 0) ok = wait_for(progress_in, need, error);
-            ln_next = line_value(t, line_in, c + 2, k);
-        }
-    }
-    if constexpr (STORE) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        if (k == 0) __hip_atomic_store(progress_out, t.len, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if constexpr (LOAD) raw_next = line_word(t, x.line_in, c + 2, k);
     }
     return h[R - 1];
 }
 
-template <int R, int NW>
-__global__ __launch_bounds__(NW * 64) void nw_long_kernel(LongArgs a)
+// Dynamic LDS of one workgroup (one wave): the strip profile, the slot ring,
+// the byte table, the code -> slot and slot -> code maps.
+template <int R>
+__global__ __launch_bounds__(kLanes) void nw_long_kernel(LongArgs a)
 {
     extern __shared__ __attribute__((aligned(16))) uint32_t lds[];   // 16-byte aligned: ds_read_b128 of profiles
     constexpr int STRIP = kLanes * R;
-    const int wave = threadIdx.x / 64;
-    const int lane = threadIdx.x % 64;
+    const int lane = threadIdx.x;
     const int prof_words = a.slots * kLanes * R / 4;
-    dev::lds_u32* base = (dev::lds_u32*)lds;
-    dev::lds_u32* prof = base + wave * prof_words;
-    volatile dev::lds_u8* ring = (dev::lds_u8*)(base + NW * prof_words) + wave * kRingBytes;
-    dev::lds_u8* tab = (dev::lds_u8*)(base + NW * prof_words) + NW * kRingBytes;
+    dev::lds_u32* prof = (dev::lds_u32*)lds;
+    volatile dev::lds_u8* ring = (dev::lds_u8*)(prof + prof_words);
+    dev::lds_u8* tab = (dev::lds_u8*)(prof + prof_words) + kRingBytes;
     dev::lds_u8* remap = tab + 640;
     dev::lds_u8* slot_code = remap + 32;
-    int* progress = (int*)(slot_code + 32);      // max_boundaries counters
-    __shared__ int s_pair;
-    __shared__ int s_result;
 
-    for (int i = threadIdx.x; i < 640; i += NW * 64) tab[i] = a.sxx.v[i];
-    if (threadIdx.x < 32) {
-        remap[threadIdx.x] = a.remap[threadIdx.x];
-        slot_code[threadIdx.x] = a.slot_code[threadIdx.x];
+    for (int i = lane; i < 640; i += kLanes) tab[i] = a.sxx.v[i];
+    if (lane < 32) {
+        remap[lane] = a.remap[lane];
+        slot_code[lane] = a.slot_code[lane];
     }
     const int null_slot = a.slots - 1;
-    int32_t* lines = a.lines + (int64_t)blockIdx.x * a.max_boundaries * a.line_stride;
-    __amdgpu_buffer_rsrc_t lines_rsrc;
-    {
-        const uint64_t wb = (uint64_t)(uintptr_t)lines;
-        const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)wb);
-        const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(wb >> 32));
-        lines_rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)(((uint64_t)hi << 32) | lo), 0,
-                                                       a.max_boundaries * a.line_stride * 4, kRawBufferFlags);
-    }
+    // Tags of this execution: tag_base + occupant, all above any earlier tag.
+    const uint32_t tag_base = *a.tag_word - (uint32_t)a.occupants;
 
     while (true) {
-        __syncthreads();
-        if (threadIdx.x == 0) s_pair = atomicAdd(a.queue, 1);
-        for (int i = threadIdx.x; i < a.max_boundaries; i += NW * 64) progress[i] = 0;
-        __syncthreads();
-        const int it = s_pair;
-        if (it >= a.npairs) break;
-        const int4 pr = a.pairs[it];
+        int t = 0;
+        if (lane == 0) t = atomicAdd(a.queue, 1);
+        t = __shfl(t, 0);
+        if (t >= a.ntasks) break;
+        const int4 task = a.tasks[t];
+        const int4 pr = a.pairs[task.x];
+        const int st = task.y;
         const SeqInfo qi = a.info[pr.x];
         const SeqInfo ti = a.info[pr.y];
         const int Lq = qi.z;
         const int nst = (Lq + STRIP - 1) / STRIP;
         const int pad_top = nst * STRIP - Lq;
-        Target t;
-        t.seq = a.blocks + ti.x;
-        t.len = ti.z;
-        t.chunks = (t.len + 2 * kLanes - 2) / kLanes;
-        t.lpad = t.chunks * kLanes - (t.len + kLanes - 1);
+        Target tg;
+        tg.seq = a.blocks + ti.x;
+        tg.len = ti.z;
+        tg.chunks = (tg.len + 2 * kLanes - 2) / kLanes;
+        tg.lpad = tg.chunks * kLanes - (tg.len + kLanes - 1);
 
-        for (int st = wave; st < nst; st += NW) {
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_s_waitcnt(0xC07F);
-            __builtin_amdgcn_wave_barrier();
-            build_profile<R>(prof, tab, slot_code, a.slots, a.blocks + qi.x, st * STRIP, pad_top, pr.w != 0, lane);
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_s_waitcnt(0xC07F);
-            __builtin_amdgcn_wave_barrier();
-            const bool load = st > 0, store = st + 1 < nst;
-            const int32_t* lin = lines + (int64_t)(st - 1) * a.line_stride + kLongLineGuard;
-            LineStore lout;
-            lout.rsrc = lines_rsrc;
-            lout.lane_off = lane == kLanes - 1 ? (uint32_t)(st * a.line_stride + kLongLineGuard) * 4u : kOutOfRange;
-            int res;
-            if (!load && !store)
-                res = run_strip<R, false, false>(prof, remap, null_slot, t, lin, progress, lout, progress, a.error, ring, lane);
-            else if (!load)
-                res = run_strip<R, false, true>(prof, remap, null_slot, t, lin, progress, lout, progress + st, a.error, ring, lane);
-            else if (store)
-                res = run_strip<R, true, true>(prof, remap, null_slot, t, lin, progress + st - 1, lout, progress + st, a.error,
-                                               ring, lane);
-            else
-                res = run_strip<R, true, false>(prof, remap, null_slot, t, lin, progress + st - 1, lout, progress, a.error,
-                                                ring, lane);
-            if (st == nst - 1 && lane == kLanes - 1) s_result = res;
+        // The previous task's LDS reads (profile, ring) are done before the
+        // profile is rebuilt.
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_s_waitcnt(0xC07F);
+        __builtin_amdgcn_wave_barrier();
+        build_profile<R>(prof, tab, slot_code, a.slots, a.blocks + qi.x, st * STRIP, pad_top, pr.w != 0, lane);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_s_waitcnt(0xC07F);
+        __builtin_amdgcn_wave_barrier();
+
+        const bool load = task.z >= 0, store = task.w >= 0;
+        const int si = load ? task.z % a.nlines : 0, ki = load ? task.z / a.nlines : 0;
+        const int so = store ? task.w % a.nlines : 0, ko = store ? task.w / a.nlines : 0;
+        Exchange x;
+        x.line_in = a.lines + (int64_t)si * a.line_stride + kLongLineGuard;
+        x.tag_in = tag_base + (uint32_t)ki;
+        x.tag_out = (int32_t)(tag_base + (uint32_t)ko);
+        {
+            const uint64_t wb = (uint64_t)(uintptr_t)(a.lines + (int64_t)so * a.line_stride);
+            const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)wb);
+            const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(wb >> 32));
+            x.out = __builtin_amdgcn_make_buffer_rsrc((void*)(((uint64_t)hi << 32) | lo), 0, a.line_stride * 8,
+                                                      kRawBufferFlags);
         }
-        __syncthreads();
-        if (threadIdx.x == 0) a.out[pr.z] = s_result - (Lq + t.len) * a.penalty;
+        // Lane u of chunk c's row holds column c*64 + u - 63 - lpad, stored at
+        // word guard + that column.
+        x.lane_off = (uint32_t)(kLongLineGuard + lane - (kLanes - 1) - tg.lpad) * 8u;
+
+        bool ok = true;
+        // The slot's previous line must have been read in full before this
+        // occupant overwrites it.
+        if (store) ok = wait_turn(a.turn + so, ko, a.error);
+        int res = 0;
+        if (ok) {
+            if (!load && !store)
+                res = run_strip<R, false, false>(prof, remap, null_slot, tg, x, a.error, ring, lane, ok);
+            else if (!load)
+                res = run_strip<R, false, true>(prof, remap, null_slot, tg, x, a.error, ring, lane, ok);
+            else if (store)
+                res = run_strip<R, true, true>(prof, remap, null_slot, tg, x, a.error, ring, lane, ok);
+            else
+                res = run_strip<R, true, false>(prof, remap, null_slot, tg, x, a.error, ring, lane, ok);
+        }
+        if (!ok) break;     // the error flag is up; every wave stops
+        if (load) {
+            // The line above has been read: its slot goes to the next occupant.
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            if (lane == 0) __hip_atomic_store(a.turn + si, ki + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // The last strip's last lane holds the last query row at the last column.
+        if (st == nst - 1 && lane == kLanes - 1) a.out[pr.z] = res - (Lq + tg.len) * a.penalty;
     }
 }
 
-__global__ void long_reset_kernel(int32_t* queue, int32_t* error)
+// Clears the queue counter, the error flag and every line slot's turn (a fill
+// kernel: no memset node in a captured graph), and moves the tag counter past
+// every tag this execution uses: occupants + 1 higher than before.
+__global__ void long_reset_kernel(int32_t* queue, int32_t* error, int32_t* turn, uint32_t* tag_word, int32_t nlines,
+                                  int32_t occupants)
 {
-    if (threadIdx.x == 0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
         queue[0] = 0;
         error[0] = 0;
+        tag_word[0] += (uint32_t)occupants + 1u;
     }
+    for (int s = i; s < nlines; s += gridDim.x * blockDim.x) turn[s] = 0;
 }
 
-template <int R, int NW>
-const void* long_fn() { return reinterpret_cast<const void*>(&nw_long_kernel<R, NW>); }
-
-constexpr int kLongWaves = 4;
+template <int R>
+const void* long_fn() { return reinterpret_cast<const void*>(&nw_long_kernel<R>); }
 
 }  // namespace
 
 int long_rows_for_slots(int slots) { return slots <= 8 ? 32 : (slots <= 16 ? 16 : 8); }
 
-int long_threads() { return kLongWaves * 64; }
-
 int long_strip_rows(int rows) { return kLanes * rows; }
 
-size_t long_lds_bytes(int rows, int slots, int max_boundaries)
+size_t long_lds_bytes(int rows, int slots)
 {
-    return (size_t)kLongWaves * (slots * kLanes * rows + kRingBytes) + 640 + 64 + 4 * (size_t)max_boundaries;
+    return (size_t)slots * kLanes * rows + kRingBytes + 640 + 64;
 }
 
 static const void* long_kernel_fn(int rows)
 {
     switch (rows) {
-    case 8: return long_fn<8, kLongWaves>();
-    case 16: return long_fn<16, kLongWaves>();
-    default: return long_fn<32, kLongWaves>();
+    case 8: return long_fn<8>();
+    case 16: return long_fn<16>();
+    default: return long_fn<32>();
     }
 }
 
@@ -383,45 +471,64 @@ int long_occupancy(int rows, size_t lds, int* blocks_per_cu)
     const void* fn = long_kernel_fn(rows);
     MSA_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int n = 0;
-    MSA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kLongWaves * 64, lds));
+    MSA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kLanes, lds));
     *blocks_per_cu = n;
     return MSA_OK;
 }
 
-int long_launch(int rows, int grid, size_t lds, const uint16_t* d_blocks, const SeqInfo* d_info, const int4* d_pairs,
-                int32_t npairs, int32_t* d_queue, int32_t* d_error, int32_t* d_out, int32_t* d_lines,
-                int32_t line_stride, int32_t max_boundaries, int slots, const uint8_t* remap,
-                const uint8_t* slot_code, const Sxx& sxx, int32_t penalty, hipStream_t stream)
+int long_max_grid(int compute_units, int* grid)
 {
-    hipLaunchKernelGGL(long_reset_kernel, dim3(1), dim3(64), 0, stream, d_queue, d_error);
+    // The fewest profile slots each strip height is used with (a compact
+    // alphabet of 1, 8 or 16 codes plus the neutral slot) gives its least LDS
+    // and so its most resident workgroups.
+    int most = 0;
+    for (const int slots : {2, 9, 17}) {
+        const int rows = long_rows_for_slots(slots);
+        int per_cu = 0;
+        MSA_TRY(long_occupancy(rows, long_lds_bytes(rows, slots), &per_cu));
+        most = std::max(most, per_cu);
+    }
+    *grid = most * compute_units;
+    return MSA_OK;
+}
+
+int long_launch(const LongLaunch& l, hipStream_t stream)
+{
+    const int reset_grid = (int)std::min<int64_t>(1024, ((int64_t)l.nlines + 255) / 256);
+    hipLaunchKernelGGL(long_reset_kernel, dim3(std::max(reset_grid, 1)), dim3(256), 0, stream, l.queue, l.error,
+                       l.turn, l.tag_word, l.nlines, l.occupants);
     MSA_HIP(hipGetLastError());
     LongArgs a;
-    a.blocks = d_blocks;
-    a.info = d_info;
-    a.pairs = d_pairs;
-    a.npairs = npairs;
-    a.queue = d_queue;
-    a.error = d_error;
-    a.out = d_out;
-    a.lines = d_lines;
-    a.line_stride = line_stride;
-    a.max_boundaries = max_boundaries;
-    a.slots = slots;
-    a.penalty = penalty;
-    a.sxx = sxx;
+    a.blocks = l.blocks;
+    a.info = l.info;
+    a.pairs = l.pairs;
+    a.tasks = l.tasks;
+    a.ntasks = l.ntasks;
+    a.queue = l.queue;
+    a.error = l.error;
+    a.out = l.out;
+    a.lines = l.lines;
+    a.turn = l.turn;
+    a.tag_word = l.tag_word;
+    a.line_stride = l.line_stride;
+    a.nlines = l.nlines;
+    a.occupants = l.occupants;
+    a.slots = l.slots;
+    a.penalty = l.penalty;
+    a.sxx = l.sxx;
     for (int i = 0; i < 32; ++i) {
-        a.remap[i] = remap[i];
-        a.slot_code[i] = slot_code[i];
+        a.remap[i] = l.remap[i];
+        a.slot_code[i] = l.slot_code[i];
     }
-    switch (rows) {
+    switch (l.rows) {
     case 8:
-        hipLaunchKernelGGL((nw_long_kernel<8, kLongWaves>), dim3(grid), dim3(kLongWaves * 64), lds, stream, a);
+        hipLaunchKernelGGL((nw_long_kernel<8>), dim3(l.grid), dim3(kLanes), l.lds, stream, a);
         break;
     case 16:
-        hipLaunchKernelGGL((nw_long_kernel<16, kLongWaves>), dim3(grid), dim3(kLongWaves * 64), lds, stream, a);
+        hipLaunchKernelGGL((nw_long_kernel<16>), dim3(l.grid), dim3(kLanes), l.lds, stream, a);
         break;
     default:
-        hipLaunchKernelGGL((nw_long_kernel<32, kLongWaves>), dim3(grid), dim3(kLongWaves * 64), lds, stream, a);
+        hipLaunchKernelGGL((nw_long_kernel<32>), dim3(l.grid), dim3(kLanes), l.lds, stream, a);
         break;
     }
     return hip_status(hipGetLastError(), "nw_long_kernel launch");

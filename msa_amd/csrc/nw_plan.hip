@@ -155,8 +155,38 @@ int device_shape(DeviceShape& out)
         MSA_HIP(hipGetDeviceProperties(&p, dev));
         cache[dev].compute_units = p.multiProcessorCount;
         cache[dev].lds_per_block = p.sharedMemPerBlock;
+        cache[dev].lds_per_cu = p.maxSharedMemoryPerMultiProcessor;
     }
     out = cache[dev];
+    return MSA_OK;
+}
+
+int resolve_tuning(const msa_nw_options& opt, const Tuning* tuning_override, Tuning& loaded, const Tuning*& out)
+{
+    out = &builtin_tuning();
+    const char* tuning_file = tuning_override ? nullptr : opt.tuning_file;
+    if (tuning_override) {
+        out = tuning_override;
+    } else if (!tuning_file) {
+        const char* env = std::getenv("MSA_NW_TUNING");
+        if (env && *env) tuning_file = env;
+    }
+    if (tuning_file) {
+        MSA_TRY(load_tuning(tuning_file, loaded));
+        out = &loaded;
+    }
+    return MSA_OK;
+}
+
+int long_threshold_of(const msa_nw_options& opt, int32_t* threshold)
+{
+    if (opt.kernel == MSA_KERNEL_LONG) return *threshold = 0, MSA_OK;
+    if (opt.kernel == MSA_KERNEL_PLAIN || opt.kernel == MSA_KERNEL_BATCHED) return *threshold = INT32_MAX, MSA_OK;
+    if (opt.long_threshold > 0) return *threshold = opt.long_threshold, MSA_OK;
+    Tuning loaded;
+    const Tuning* tuning = nullptr;
+    MSA_TRY(resolve_tuning(opt, nullptr, loaded, tuning));
+    *threshold = tuning->long_threshold;
     return MSA_OK;
 }
 
@@ -186,18 +216,8 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
             return fail(MSA_ERR_INVALID_ARGUMENT, "no batched tile of %d lanes x %d rows per lane", lanes, rows);
     }
     Tuning loaded;
-    const Tuning* tuning = &builtin_tuning();
-    const char* tuning_file = tuning_override ? nullptr : opt.tuning_file;
-    if (tuning_override) {
-        tuning = tuning_override;
-    } else if (!tuning_file) {
-        const char* env = std::getenv("MSA_NW_TUNING");
-        if (env && *env) tuning_file = env;
-    }
-    if (tuning_file) {
-        MSA_TRY(load_tuning(tuning_file, loaded));
-        tuning = &loaded;
-    }
+    const Tuning* tuning = nullptr;
+    MSA_TRY(resolve_tuning(opt, tuning_override, loaded, tuning));
 
     std::vector<HostSeq> seqs;
     int64_t total_blocks = 0;
@@ -492,7 +512,7 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
     // from the profile size, one pair per workgroup queued longest first.
     if (!longs.empty()) {
         bool present[32] = {};
-        int32_t max_t = 1, max_q = 1;
+        int32_t max_t = 1;
         for (const auto& f : longs) {
             for (const int32_t sidx : {f.query >> 1, f.target}) {
                 const HostSeq& h = seqs[sidx];
@@ -500,7 +520,6 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
                     present[(blocks[h.offset + pos / 3] >> (1 + 5 * (pos % 3))) & 0x1F] = true;
             }
             max_t = std::max(max_t, seqs[f.target].first_end);
-            max_q = std::max(max_q, seqs[f.query >> 1].first_end);
         }
         int slots = 0;
         for (int code = 0; code < 32; ++code) hp.slot_code[code] = 0xFF;
@@ -515,22 +534,88 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
             if (code >= kEndCode || !present[code]) hp.remap[code] = (uint8_t)slots;
         hp.long_rows = long_rows_for_slots(hp.long_slots);
         const int strip = long_strip_rows(hp.long_rows);
-        hp.long_boundaries = std::max(1, (max_q + strip - 1) / strip - 1);
         hp.long_stride = long_line_stride(max_t);
-        hp.long_lds = long_lds_bytes(hp.long_rows, hp.long_slots, hp.long_boundaries);
+        // One buffer descriptor spans one line of 8-byte words: its byte size
+        // must fit 31 bits.
+        if ((int64_t)hp.long_stride * 8 >= (int64_t(1) << 31))
+            return fail(MSA_ERR_TOO_LARGE, "a long pair's target of %d residues exceeds the boundary line's "
+                        "limit of %d", max_t, (int)((int64_t(1) << 28) - kLongLineGuard - 64));
+        hp.long_lds = long_lds_bytes(hp.long_rows, hp.long_slots);
         if (hp.long_lds > dev.lds_per_block)
             return fail(MSA_ERR_TOO_LARGE, "long pairs need %zu bytes of LDS per workgroup", hp.long_lds);
         int per_cu = 0;
         MSA_TRY(long_occupancy(hp.long_rows, hp.long_lds, &per_cu));
         if (per_cu < 1) return fail(MSA_ERR_RUNTIME, "long kernel does not fit a compute unit");
+        // Resident one-wave workgroups per CU: a multiple of 4, so every SIMD
+        // holds as many waves and the strips of a pair run at one speed; and
+        // about one per long pair (at least 2 per SIMD), so that a strip's
+        // producer is about a round of the grid ahead of it and the strips do
+        // not run in lockstep, each waiting on the one above at every chunk
+        // (measured on 64 x 30,000 DNA: 14 per CU 139 ms, 12 per CU 122 ms,
+        // 8 per CU 110 ms).  Residency is capped by reserving LDS.
+        if (per_cu >= 8 && dev.lds_per_cu > 0) {
+            const int64_t per_pair = ((int64_t)longs.size() + dev.compute_units - 1) / dev.compute_units;
+            const int want = (int)std::min<int64_t>(per_cu / 4 * 4, std::max<int64_t>(8, (per_pair + 3) / 4 * 4));
+            const size_t capped = dev.lds_per_cu / (size_t)want / 512 * 512;
+            if (want < per_cu && capped >= hp.long_lds && capped <= dev.lds_per_block) {
+                int got = 0;
+                MSA_TRY(long_occupancy(hp.long_rows, capped, &got));
+                if (got == want) {
+                    hp.long_lds = capped;
+                    per_cu = got;
+                }
+            }
+        }
         std::vector<std::pair<int64_t, int4>> order;
         for (const auto& f : longs)
             order.push_back({(int64_t)seqs[f.query >> 1].first_end * seqs[f.target].first_end,
                              make_int4(f.query >> 1, f.target, f.index, f.query & 1)});
         std::stable_sort(order.begin(), order.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
         hp.long_pairs.resize(order.size());
-        for (size_t u = 0; u < order.size(); ++u) hp.long_pairs[u] = order[u].second;
-        hp.long_grid = (int)std::min<int64_t>((int64_t)per_cu * dev.compute_units, (int64_t)hp.long_pairs.size());
+        int64_t ntasks = 0;
+        std::vector<int32_t> nst(order.size());
+        for (size_t u = 0; u < order.size(); ++u) {
+            hp.long_pairs[u] = order[u].second;
+            nst[u] = (seqs[order[u].second.x].first_end + strip - 1) / strip;
+            ntasks += nst[u];
+        }
+        if (ntasks >= (int64_t(1) << 31))
+            return fail(MSA_ERR_TOO_LARGE, "long pairs make %lld strips; at most 2^31-1", (long long)ntasks);
+        hp.long_grid = (int)std::min<int64_t>((int64_t)per_cu * dev.compute_units, ntasks);
+        // Tasks: the pairs (costliest first) in groups as large as the grid,
+        // each group level by level, so that the strip above a task comes
+        // about a group earlier in the list (groups of a quarter of the grid
+        // ran 5% slower).  Strips with a strip below them get the next line
+        // number; a task records the line it reads (its producer's) and the
+        // line it writes.
+        const int64_t group = std::max<int64_t>(1, (int64_t)hp.long_grid);
+        hp.long_tasks.reserve((size_t)ntasks);
+        std::vector<int32_t> line_of(order.size(), -1);   // the last line each pair wrote
+        std::vector<int64_t> task_of_line;                // task index of every line's writer
+        int64_t max_gap = 0;                              // longest writer -> reader distance in the list
+        int32_t nlines_used = 0;
+        for (size_t g0 = 0; g0 < order.size(); g0 += (size_t)group) {
+            const size_t g1 = std::min(order.size(), g0 + (size_t)group);
+            int32_t levels = 0;
+            for (size_t u = g0; u < g1; ++u) levels = std::max(levels, nst[u]);
+            for (int32_t s = 0; s < levels; ++s)
+                for (size_t u = g0; u < g1; ++u) {
+                    if (s >= nst[u]) continue;
+                    const int32_t in = s > 0 ? line_of[u] : -1;
+                    const int32_t out = s + 1 < nst[u] ? nlines_used++ : -1;
+                    const int64_t here = (int64_t)hp.long_tasks.size();
+                    if (in >= 0) max_gap = std::max(max_gap, here - task_of_line[in]);
+                    if (out >= 0) task_of_line.push_back(here);
+                    line_of[u] = out;
+                    hp.long_tasks.push_back(make_int4((int)u, s, in, out));
+                }
+        }
+        // A writer waits for the reader of the line nlines numbers before its
+        // own; that reader comes earlier in the list when the ring is longer
+        // than every writer -> reader distance.
+        hp.long_nlines = (int32_t)std::max<int64_t>(1, std::min<int64_t>(
+            std::max<int64_t>((int64_t)kLongLinesPerWave * hp.long_grid + group, max_gap + 1), nlines_used));
+        hp.long_occupants = nlines_used > 0 ? (nlines_used + hp.long_nlines - 1) / hp.long_nlines : 0;
     }
     int batched_launches = 0;
     for (int c = 0; c < kTileConfigCount; ++c) batched_launches += hp.batch_grid[c] > 0 ? 2 : 0;
@@ -551,6 +636,7 @@ DataLayout data_layout(const HostPlan& hp)
     place(lay.items, hp.items.size() * sizeof(int4));
     place(lay.targets, hp.targets.size() * sizeof(int2));
     place(lay.long_pairs, hp.long_pairs.size() * sizeof(int4));
+    place(lay.long_tasks, hp.long_tasks.size() * sizeof(int4));
     lay.bytes = at;
     return lay;
 }
@@ -579,6 +665,7 @@ void pack_data(const HostPlan& hp, const DataLayout& lay, void* host_block)
     put(lay.items, hp.items.data(), hp.items.size() * sizeof(int4));
     put(lay.targets, hp.targets.data(), hp.targets.size() * sizeof(int2));
     put(lay.long_pairs, hp.long_pairs.data(), hp.long_pairs.size() * sizeof(int4));
+    put(lay.long_tasks, hp.long_tasks.data(), hp.long_tasks.size() * sizeof(int4));
 }
 
 ScratchNeed scratch_need(const HostPlan& hp)
@@ -592,22 +679,30 @@ ScratchNeed scratch_need(const HostPlan& hp)
         const int grid = *std::max_element(hp.batch_grid, hp.batch_grid + kTileConfigCount);
         n.batch_lines = up64((size_t)grid * batched_groups_per_block() * hp.batch_stride);
     }
-    if (!hp.long_pairs.empty()) n.long_lines = up64((size_t)hp.long_grid * hp.long_boundaries * hp.long_stride);
+    if (!hp.long_pairs.empty()) {
+        n.long_lines = up64((size_t)2 * hp.long_nlines * hp.long_stride);
+        n.long_slots = up64((size_t)hp.long_nlines);
+    }
     return n;
 }
 
 Scratch carve_scratch(const ScratchNeed& need, int32_t* base)
 {
+    // Every region is a multiple of 64 ints, so each starts 256-byte aligned.
     Scratch s;
     s.batch_queue = base;
     s.long_queue = base + 64;
     s.long_error = base + 128;
-    int32_t* at = base + 192;
+    s.long_tag = reinterpret_cast<uint32_t*>(base + 192);
+    int32_t* at = base + 256;
+    s.long_turn = at;
+    at += need.long_slots;
     s.plain_lines = at;
     at += need.plain_lines;
     s.batch_lines = at;
     at += need.batch_lines;
     s.long_lines = at;
+    s.long_lines64 = reinterpret_cast<uint64_t*>(at);
     return s;
 }
 
@@ -625,11 +720,32 @@ int launch_plan(const HostPlan& hp, const DataLayout& lay, const void* d_data, c
                                reinterpret_cast<const int2*>(base + lay.targets), scr.batch_queue + 16 * c, d_out,
                                scr.batch_lines, std::max(hp.batch_stride, 1), hp.sxx, hp.penalty, stream));
     }
-    if (!hp.long_pairs.empty())
-        MSA_TRY(long_launch(hp.long_rows, hp.long_grid, hp.long_lds, d_blocks, d_info,
-                            reinterpret_cast<const int4*>(base + lay.long_pairs), (int32_t)hp.long_pairs.size(),
-                            scr.long_queue, scr.long_error, d_out, scr.long_lines, hp.long_stride,
-                            hp.long_boundaries, hp.long_slots, hp.remap, hp.slot_code, hp.sxx, hp.penalty, stream));
+    if (!hp.long_pairs.empty()) {
+        LongLaunch l;
+        l.rows = hp.long_rows;
+        l.grid = hp.long_grid;
+        l.slots = hp.long_slots;
+        l.lds = hp.long_lds;
+        l.blocks = d_blocks;
+        l.info = d_info;
+        l.pairs = reinterpret_cast<const int4*>(base + lay.long_pairs);
+        l.tasks = reinterpret_cast<const int4*>(base + lay.long_tasks);
+        l.ntasks = (int32_t)hp.long_tasks.size();
+        l.queue = scr.long_queue;
+        l.error = scr.long_error;
+        l.out = d_out;
+        l.lines = scr.long_lines64;
+        l.turn = scr.long_turn;
+        l.tag_word = scr.long_tag;
+        l.line_stride = hp.long_stride;
+        l.nlines = hp.long_nlines;
+        l.occupants = hp.long_occupants;
+        l.remap = hp.remap;
+        l.slot_code = hp.slot_code;
+        l.sxx = hp.sxx;
+        l.penalty = hp.penalty;
+        MSA_TRY(long_launch(l, stream));
+    }
     if (!hp.plain_index.empty())
         MSA_TRY(plain_launch(d_blocks, d_info, reinterpret_cast<const int32_t*>(base + lay.plain_pairs),
                              reinterpret_cast<const int32_t*>(base + lay.plain_index),
@@ -709,6 +825,11 @@ int msa::create_plan(msa_nw_plan* plan, const uint16_t* blocks, const int64_t* o
             rc = msa::hip_status(hipMalloc(&p->d_data, p->lay.bytes), "hipMalloc");
             if (rc == MSA_OK)
                 rc = msa::hip_status(hipMalloc(&p->d_scratch, p->need.words() * sizeof(int32_t)), "hipMalloc");
+            // The scratch starts zeroed (the long kernel's tag counter and
+            // lines); this is plan creation, never inside a captured graph.
+            if (rc == MSA_OK)
+                rc = msa::hip_status(hipMemsetAsync(p->d_scratch, 0, p->need.words() * sizeof(int32_t), stream),
+                                     "hipMemsetAsync");
             if (rc == MSA_OK)
                 rc = msa::hip_status(hipMemcpyAsync(p->d_data, staging.get(), p->lay.bytes, hipMemcpyHostToDevice,
                                                     stream), "hipMemcpyAsync");

@@ -55,12 +55,43 @@ int batched_launch(int config, int grid, const uint16_t* d_blocks, const SeqInfo
                    int32_t line_stride, const Sxx& sxx, int32_t penalty, hipStream_t stream);
 int long_rows_for_slots(int slots);
 int long_strip_rows(int rows);
-size_t long_lds_bytes(int rows, int slots, int max_boundaries);
+size_t long_lds_bytes(int rows, int slots);
 int long_occupancy(int rows, size_t lds, int* blocks_per_cu);
-int long_launch(int rows, int grid, size_t lds, const uint16_t* d_blocks, const SeqInfo* d_info, const int4* d_pairs,
-                int32_t npairs, int32_t* d_queue, int32_t* d_error, int32_t* d_out, int32_t* d_lines,
-                int32_t line_stride, int32_t max_boundaries, int slots, const uint8_t* remap,
-                const uint8_t* slot_code, const Sxx& sxx, int32_t penalty, hipStream_t stream);
+// One persistent launch of the long kernel (plus the reset of its queue,
+// error flag, slot turns and tag counter): one-wave workgroups take the
+// (pair, strip) tasks in list order; strip boundaries go through a ring of
+// `nlines` lines of 64-bit (tag, score) words.
+struct LongLaunch {
+    int rows = 32, grid = 0, slots = 0;
+    size_t lds = 0;
+    const uint16_t* blocks = nullptr;
+    const SeqInfo* info = nullptr;
+    const int4* pairs = nullptr;      // {query, target, output index, transpose flag}
+    const int4* tasks = nullptr;      // {pair, strip, line read or -1, line written or -1}
+    int32_t ntasks = 0;
+    int32_t* queue = nullptr;
+    int32_t* error = nullptr;
+    int32_t* out = nullptr;
+    uint64_t* lines = nullptr;
+    int32_t* turn = nullptr;
+    uint32_t* tag_word = nullptr;     // grows by occupants + 1 per execution; starts at 0
+    int32_t line_stride = 0, nlines = 0, occupants = 0;
+    const uint8_t* remap = nullptr;
+    const uint8_t* slot_code = nullptr;
+    Sxx sxx;
+    int32_t penalty = 0;
+};
+int long_launch(const LongLaunch& launch, hipStream_t stream);
+// Most resident long workgroups on this device for any strip height.
+int long_max_grid(int compute_units, int* grid);
+// Line slots per resident long workgroup (plus one group of pairs): an
+// occupant rarely waits for the slot's previous line to be read.
+constexpr int kLongLinesPerWave = 2;
+// Line slots a plan's long kernel uses at most, for a grid of `grid`: the
+// planner's ring is kLongLinesPerWave per workgroup plus one group (as many
+// pairs as the grid), and a group bounds the distance from a strip to the one
+// below.
+inline int64_t long_max_lines(int64_t grid) { return kLongLinesPerWave * grid + grid + 1; }
 
 constexpr int kDefaultLongThreshold = 8192;
 
@@ -79,10 +110,18 @@ const Tuning& builtin_tuning();
 // Reads a table written by msa_nw_tune (nw_tune.hip).
 int load_tuning(const char* path, Tuning& out);
 int tuning_config_for(const Tuning& t, int32_t query_len);
+// The table plans take under `opt`: `tuning_override` when not null, else the
+// file the options or MSA_NW_TUNING name (read into `loaded`), else the
+// built-in table.
+int resolve_tuning(const msa_nw_options& opt, const Tuning* tuning_override, Tuning& loaded, const Tuning*& out);
+// Shorter-side length from which plans under `opt` route clean pairs to the
+// long kernel (0 under MSA_KERNEL_LONG, INT32_MAX when they never do).
+int long_threshold_of(const msa_nw_options& opt, int32_t* threshold);
 
 struct DeviceShape {
     int compute_units = 0;
     size_t lds_per_block = 0;
+    size_t lds_per_cu = 0;
 };
 // Properties of the current device (cached per device).
 int device_shape(DeviceShape& out);
@@ -119,9 +158,10 @@ struct HostPlan {
     int32_t batch_stride = 0;
     // long kernel
     std::vector<int4> long_pairs;
+    std::vector<int4> long_tasks;         // {pair, strip, line read, line written} per strip
     int long_rows = 32, long_slots = 0, long_grid = 0;
     size_t long_lds = 0;
-    int32_t long_stride = 0, long_boundaries = 0;
+    int32_t long_stride = 0, long_nlines = 0, long_occupants = 0;   // stride in 64-bit words
     uint8_t remap[32] = {};
     uint8_t slot_code[32] = {};
 };
@@ -144,16 +184,20 @@ int create_plan(msa_nw_plan* plan, const uint16_t* blocks, const int64_t* offset
 
 // The HostPlan's arrays inside one block of memory (offsets 256-aligned).
 struct DataLayout {
-    size_t blocks = 0, seqinfo = 0, plain_pairs = 0, plain_index = 0, items = 0, targets = 0, long_pairs = 0;
+    size_t blocks = 0, seqinfo = 0, plain_pairs = 0, plain_index = 0, items = 0, targets = 0, long_pairs = 0,
+           long_tasks = 0;
     size_t bytes = 0;
 };
 DataLayout data_layout(const HostPlan& hp);
 void pack_data(const HostPlan& hp, const DataLayout& lay, void* host_block);
 
 // Scratch one execution needs (int32 words), laid out the same way.
+// The whole block is zeroed once when it is allocated (the long kernel's tag
+// counter and lines must start at 0); executions clear what they need.
 struct ScratchNeed {
-    size_t plain_lines = 0, batch_lines = 0, long_lines = 0;
-    size_t words() const { return plain_lines + batch_lines + long_lines + 3 * 64; }
+    size_t plain_lines = 0, batch_lines = 0, long_lines = 0;   // long_lines in 32-bit words (2 per line word)
+    size_t long_slots = 0;      // line slots of the long kernel: one turn each
+    size_t words() const { return plain_lines + batch_lines + long_lines + long_slots + 4 * 64; }
 };
 ScratchNeed scratch_need(const HostPlan& hp);
 
@@ -164,6 +208,9 @@ struct Scratch {
     int32_t* batch_queue = nullptr;
     int32_t* long_queue = nullptr;
     int32_t* long_error = nullptr;
+    uint32_t* long_tag = nullptr;
+    int32_t* long_turn = nullptr;
+    uint64_t* long_lines64 = nullptr;
 };
 Scratch carve_scratch(const ScratchNeed& need, int32_t* base);
 

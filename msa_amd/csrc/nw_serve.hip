@@ -298,8 +298,8 @@ int msa_srv_open(msa_server* srv, const msa_srv_config* cfg)
 
     // Scratch for the largest job the configuration admits: every line a
     // kernel touches is at most max_length long, and the grids are bounded by
-    // the device's occupancy (batched), 16 workgroups per CU (plain) and four
-    // long workgroups per CU with strips of at least 512 rows (long).
+    // the device's occupancy (batched), 16 workgroups per CU (plain) and 32
+    // one-wave workgroups per CU (long).
     {
         msa::DeviceShape dev;
         MSA_TRY(msa::device_shape(dev));
@@ -311,7 +311,17 @@ int msa_srv_open(msa_server* srv, const msa_srv_config* cfg)
         n.batch_lines = (size_t)per_cu * dev.compute_units * msa::batched_groups_per_block() *
                         (size_t)msa::batched_line_stride((int64_t)L);
         n.plain_lines = (size_t)msa::plain_grid(int64_t(1) << 40, dev.compute_units) * stride;
-        n.long_lines = (size_t)4 * dev.compute_units * ((L + 511) / 512) * (size_t)msa::long_line_stride((int64_t)L);
+        // Long kernel, only when a job of max_length residues can route a pair
+        // there: the planner's line ring for the largest grid, lines of 64-bit
+        // words.
+        int32_t threshold = 0;
+        MSA_TRY(msa::long_threshold_of(s->cfg.options, &threshold));
+        if ((int64_t)s->cfg.max_length >= threshold) {
+            int grid = 0;
+            MSA_TRY(msa::long_max_grid(dev.compute_units, &grid));
+            n.long_slots = (size_t)msa::long_max_lines(grid);
+            n.long_lines = 2 * n.long_slots * (size_t)msa::long_line_stride((int64_t)L);
+        }
         s->slot_need = n;
     }
 
@@ -325,6 +335,8 @@ int msa_srv_open(msa_server* srv, const msa_srv_config* cfg)
         MSA_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
         sl.need = s->slot_need;
         MSA_HIP(hipMalloc(&sl.scratch, sl.need.words() * sizeof(int32_t)));
+        // Zeroed once: the long kernel's tag counter and lines start at 0.
+        MSA_HIP(hipMemset(sl.scratch, 0, sl.need.words() * sizeof(int32_t)));
     }
     s->t0 = std::chrono::steady_clock::now();
     s->scheduler = std::thread([p = s.get()] { p->run(); });
@@ -350,7 +362,7 @@ int msa_srv_submit(msa_server srv, const uint16_t* blocks, const int64_t* offset
     job->lay = msa::data_layout(job->hp);
     const msa::ScratchNeed need = msa::scratch_need(job->hp);
     if (need.plain_lines > srv->slot_need.plain_lines || need.batch_lines > srv->slot_need.batch_lines ||
-        need.long_lines > srv->slot_need.long_lines)
+        need.long_lines > srv->slot_need.long_lines || need.long_slots > srv->slot_need.long_slots)
         return fail(MSA_ERR_TOO_LARGE, "job needs more kernel scratch than a server slot holds");
     const size_t bytes = job->lay.bytes + (size_t)npairs * sizeof(int32_t);
     if (bytes > srv->cfg.device_bytes || bytes + 512 > srv->cfg.host_bytes)
