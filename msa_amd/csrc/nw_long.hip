@@ -20,11 +20,10 @@
 //
 // A wave runs its strip as a 64-lane wavefront (lane k owns R rows, bottom
 // scores move one lane per step by DPP wave shifts, each lane reads its
-// column's slot from a ring in LDS) over 64-column chunks.  The strip's
-// bottom cells (the last lane's, one per step) gather in a register that
-// moves down one lane per step, and every chunk ends with one coalesced store
-// of those 64 cells into a line in global memory; the strip below reads the
-// line two chunks ahead.  Every line word is 64
+// column's slot from a ring in LDS) over 64-column chunks.  The last lane
+// stores the strip's bottom cell at every step into a line in global memory;
+// the strip below loads the line two chunks ahead and hands it to its first
+// lane through a small LDS ring.  Every line word is 64
 // bits, (tag << 32) | score, stored with the cache policy of an agent-scope
 // atomic store (sc1) and loaded by relaxed agent-scope atomic loads, so the
 // words carry their own synchronisation: a reader takes a word only when its
@@ -65,6 +64,7 @@ namespace {
 
 constexpr int kLanes = 64;
 constexpr int kRingBytes = 4 * kLanes;   // the wave's slot ring
+constexpr int kLineRingBytes = 2 * kLanes * 4;   // the wave's line ring: two chunks of ints
 constexpr int kSpinLimit = 1 << 24;     // ~1e9 cycles of waiting before giving up
 constexpr int kSc1 = 16;                // buffer-instruction cache policy: sc1 (agent scope)
 
@@ -230,9 +230,11 @@ __device__ __forceinline__ bool take_word(const Target& t, const uint64_t* line,
 // The reads are volatile so that they stay single-byte reads.
 // The bottom-row store of a strip: the strip's line as one raw buffer
 // (wave-uniform descriptor; dword 3: 32-bit data format, as for any raw
-// buffer access on gfx9), one 8-byte word per lane and chunk.  One descriptor
-// spans one line, so every offset fits in 32 bits.
+// buffer access on gfx9) and this lane's byte offset in it -- out of range,
+// so the store is dropped, on every lane but the last.  One descriptor spans
+// one line, so every offset fits in 32 bits.
 constexpr int kRawBufferFlags = 0x00020000;
+constexpr uint32_t kOutOfRange = 0x80000000u;
 typedef int v2i32 __attribute__((ext_vector_type(2)));
 
 // Where a strip reads the line above and writes its own.
@@ -240,13 +242,22 @@ struct Exchange {
     const uint64_t* line_in;    // column 0 of the line above
     uint32_t tag_in;            // its producer's tag
     __amdgpu_buffer_rsrc_t out; // this strip's line
-    uint32_t lane_off;          // byte offset of this lane's word of chunk 0 (column lane - 63 - lpad)
+    uint32_t lane_off;          // byte offset of column -63-lpad on the last lane, else out of range
     int32_t tag_out;
 };
 
+// The line above, chunk by chunk, for the first lane: every lane writes its
+// word of chunk c into an LDS ring of two chunks (64 ints each), and at step u
+// the wave reads int u of the chunk (one address for all lanes, read one step
+// ahead); the DPP shift that hands each lane the bottom cell of the lane
+// before it writes into that register in place, so the first lane keeps the
+// line value as its cell above.
+typedef __attribute__((address_space(3))) int lds_i32;
+
 template <int R, bool LOAD, bool STORE>
 __device__ int run_strip(const dev::lds_u32* prof, const dev::lds_u8* remap, int null_slot, const Target& t,
-                         const Exchange& x, int32_t* error, volatile dev::lds_u8* ring, int k, bool& ok)
+                         const Exchange& x, int32_t* error, volatile dev::lds_u8* ring, volatile lds_i32* lline, int k,
+                         bool& ok)
 {
     constexpr int W = R / 4;
     constexpr int SLOT_WORDS = kLanes * W;
@@ -265,14 +276,20 @@ __device__ int run_strip(const dev::lds_u32* prof, const dev::lds_u8* remap, int
         ring[RING + k] = c0;
     }
     int cc_next = col_slot(t, remap, null_slot, 1, k);
-    int ln = 0, ln_next = 0;
+    int lv = 0;     // the line value of the current step (first lane)
     uint64_t raw_next = 0;
     if constexpr (LOAD) {
         const uint64_t raw0 = line_word(t, x.line_in, 0, k);
         raw_next = line_word(t, x.line_in, 1, k);
-        ok = take_word(t, x.line_in, 0, k, raw0, x.tag_in, error, ln);
+        int l0 = 0;
+        ok = take_word(t, x.line_in, 0, k, raw0, x.tag_in, error, l0);
+        lline[k] = l0;
+        lv = lline[0];
     }
-    int row = 0;
+    // The last lane stores its bottom cell and the tag at every step: one
+    // 8-byte buffer store with the sc1 policy at an offset out of range (so
+    // dropped) on every other lane, the step's 8u in the scalar offset.
+    uint32_t voff = 0;
     // Slots of steps 0 and 1 (virtual columns -k and 1-k).
     const int base0 = (RING - k) & (RING - 1);
     int b = ring[base0];
@@ -280,21 +297,28 @@ __device__ int run_strip(const dev::lds_u32* prof, const dev::lds_u8* remap, int
     uint32_t w[W];
     load_prof<R>(w, prof + b * SLOT_WORDS, k);
 
-    // Step u of a chunk whose slot for step u + 2 sits at byte base + u; LAST:
-    // the step that moves to the next chunk.
-    auto step = [&](auto last_tag, int base, int u) {
+    // Step u of a chunk whose slot for step u + 2 sits at byte base + u and
+    // whose line values sit at lcur (the next chunk's at lnext); LAST: the
+    // step that moves to the next chunk.
+    auto step = [&](auto last_tag, int base, int u, volatile lds_i32* lcur, volatile lds_i32* lnext) {
         constexpr bool LAST = decltype(last_tag)::value;
         const int bnn = ring[base + u];
         uint32_t wn[W];
         load_prof<R>(wn, prof + bn * SLOT_WORDS, k);
-        const int hup = LOAD ? dev::dpp<dev::kWaveShr1>(ln, h[R - 1]) : dev::dpp_zero<dev::kWaveShr1>(h[R - 1]);
-        if constexpr (LOAD) ln = LAST ? ln_next : dev::dpp<dev::kWaveShl1>(ln, ln);
+        int hup;
+        if constexpr (LOAD) {
+            const int lvn = LAST ? lnext[0] : lcur[u + 1];
+            hup = dev::dpp<dev::kWaveShr1>(lv, h[R - 1]);
+            lv = lvn;
+        } else {
+            hup = dev::dpp_zero<dev::kWaveShr1>(h[R - 1]);
+        }
         dev::cells<R>(h, hup, dtop, w);
         dtop = hup;
-        // The last lane's bottom cell enters the chunk's row collector, which
-        // moves down one lane per step: after step 63, lane u holds step u's
-        // cell (column c*64 + u - 63 - lpad).
-        if constexpr (STORE) row = dev::dpp<dev::kWaveShl1>(h[R - 1], row);
+        if constexpr (STORE) {
+            const v2i32 word = {h[R - 1], x.tag_out};
+            __builtin_amdgcn_raw_buffer_store_b64(word, x.out, (int)voff, 8 * u, kSc1);
+        }
         b = bn;
         bn = bnn;
 #pragma unroll
@@ -303,28 +327,29 @@ __device__ int run_strip(const dev::lds_u32* prof, const dev::lds_u8* remap, int
 
     for (int c = 0; c < t.chunks && ok; ++c) {
         const int base = (c * kLanes + 2 - k) & (RING - 1);
+        volatile lds_i32* lcur = lline + (c & 1) * kLanes;
+        volatile lds_i32* lnext = lline + ((c + 1) & 1) * kLanes;
+        // Step u stores column c*64 + u - 63 - lpad (last lane).
+        if constexpr (STORE) voff = x.lane_off + (uint32_t)(c * kLanes) * 8u;
 #pragma unroll 9
-        for (int u = 0; u < kLanes - 3; ++u) step(std::false_type {}, base, u);
+        for (int u = 0; u < kLanes - 3; ++u) step(std::false_type {}, base, u, lcur, lnext);
         // Chunk c+1 replaces chunk c-1, whose last reader was step 60.
         {
             const int slot = ((c + 1) * kLanes + k) & (RING - 1);
             ring[slot] = cc_next;
             ring[RING + slot] = cc_next;
         }
-        step(std::false_type {}, base, kLanes - 3);
-        step(std::false_type {}, base, kLanes - 2);
-        // The line above for chunk c+1, loaded a chunk ago, is taken before
-        // the step that moves to it.
-        if constexpr (LOAD) ok = take_word(t, x.line_in, c + 1, k, raw_next, x.tag_in, error, ln_next);
-        step(std::true_type {}, base, kLanes - 1);
-        // The chunk's bottom row and the tag: one 512-byte store of 8-byte
-        // words with the sc1 policy; columns left of the target land in the
-        // guard.
-        if constexpr (STORE) {
-            const v2i32 word = {row, x.tag_out};
-            __builtin_amdgcn_raw_buffer_store_b64(word, x.out, (int)(x.lane_off + (uint32_t)(c * kLanes) * 8u), 0,
-                                                  kSc1);
+        step(std::false_type {}, base, kLanes - 3, lcur, lnext);
+        step(std::false_type {}, base, kLanes - 2, lcur, lnext);
+        // The line above for chunk c+1, loaded a chunk ago, is taken and put
+        // in the LDS ring (over chunk c-1, last read at step 62 of chunk c-1)
+        // before the step that reads its first value.
+        if constexpr (LOAD) {
+            int lnv = 0;
+            ok = take_word(t, x.line_in, c + 1, k, raw_next, x.tag_in, error, lnv);
+            lnext[k] = lnv;
         }
+        step(std::true_type {}, base, kLanes - 1, lcur, lnext);
         cc_next = col_slot(t, remap, null_slot, c + 2, k);
         if constexpr (LOAD) raw_next = line_word(t, x.line_in, c + 2, k);
     }
@@ -332,7 +357,7 @@ __device__ int run_strip(const dev::lds_u32* prof, const dev::lds_u8* remap, int
 }
 
 // Dynamic LDS of one workgroup (one wave): the strip profile, the slot ring,
-// the byte table, the code -> slot and slot -> code maps.
+// the line ring, the byte table, the code -> slot and slot -> code maps.
 template <int R>
 __global__ __launch_bounds__(kLanes) void nw_long_kernel(LongArgs a)
 {
@@ -342,7 +367,8 @@ __global__ __launch_bounds__(kLanes) void nw_long_kernel(LongArgs a)
     const int prof_words = a.slots * kLanes * R / 4;
     dev::lds_u32* prof = (dev::lds_u32*)lds;
     volatile dev::lds_u8* ring = (dev::lds_u8*)(prof + prof_words);
-    dev::lds_u8* tab = (dev::lds_u8*)(prof + prof_words) + kRingBytes;
+    volatile lds_i32* lline = (lds_i32*)(prof + prof_words + kRingBytes / 4);
+    dev::lds_u8* tab = (dev::lds_u8*)(prof + prof_words) + kRingBytes + kLineRingBytes;
     dev::lds_u8* remap = tab + 640;
     dev::lds_u8* slot_code = remap + 32;
 
@@ -400,7 +426,7 @@ __global__ __launch_bounds__(kLanes) void nw_long_kernel(LongArgs a)
         }
         // Lane u of chunk c's row holds column c*64 + u - 63 - lpad, stored at
         // word guard + that column.
-        x.lane_off = (uint32_t)(kLongLineGuard + lane - (kLanes - 1) - tg.lpad) * 8u;
+        x.lane_off = lane == kLanes - 1 ? (uint32_t)(kLongLineGuard - (kLanes - 1) - tg.lpad) * 8u : kOutOfRange;
 
         bool ok = true;
         // The slot's previous line must have been read in full before this
@@ -409,13 +435,13 @@ __global__ __launch_bounds__(kLanes) void nw_long_kernel(LongArgs a)
         int res = 0;
         if (ok) {
             if (!load && !store)
-                res = run_strip<R, false, false>(prof, remap, null_slot, tg, x, a.error, ring, lane, ok);
+                res = run_strip<R, false, false>(prof, remap, null_slot, tg, x, a.error, ring, lline, lane, ok);
             else if (!load)
-                res = run_strip<R, false, true>(prof, remap, null_slot, tg, x, a.error, ring, lane, ok);
+                res = run_strip<R, false, true>(prof, remap, null_slot, tg, x, a.error, ring, lline, lane, ok);
             else if (store)
-                res = run_strip<R, true, true>(prof, remap, null_slot, tg, x, a.error, ring, lane, ok);
+                res = run_strip<R, true, true>(prof, remap, null_slot, tg, x, a.error, ring, lline, lane, ok);
             else
-                res = run_strip<R, true, false>(prof, remap, null_slot, tg, x, a.error, ring, lane, ok);
+                res = run_strip<R, true, false>(prof, remap, null_slot, tg, x, a.error, ring, lline, lane, ok);
         }
         if (!ok) break;     // the error flag is up; every wave stops
         if (load) {
@@ -454,7 +480,7 @@ int long_strip_rows(int rows) { return kLanes * rows; }
 
 size_t long_lds_bytes(int rows, int slots)
 {
-    return (size_t)slots * kLanes * rows + kRingBytes + 640 + 64;
+    return (size_t)slots * kLanes * rows + kRingBytes + kLineRingBytes + 640 + 64;
 }
 
 static const void* long_kernel_fn(int rows)
