@@ -278,6 +278,34 @@ int msa_nj_build(const float* distances, int32_t n, int32_t variant, msa_nj_node
 int msa_nj_build_device(const int32_t* d_scores, int32_t n, int32_t variant, msa_nj_node* d_nodes,
                         msa_stream stream);
 
+/* ---- profile alignment (pgalign) ----------------------------------------------
+ * The stage after the guide tree: module::pgalign::run and pgalign::run
+ * (src/pgalign.cpp:25-47, src/pgalign/pgalign.cuh:58-90), whose only
+ * algorithm, "myers", is a stub returning an empty alignment
+ * (src/pgalign/myers/impl/sequential.cpp:29-32).  Here the stage computes the
+ * multiple alignment: every join of the guide tree, children first, aligns
+ * the two children's alignments optimally under the sum-of-pairs score with
+ * the pairwise table and linear gap penalty (gap against gap scores 0); for two
+ * single sequences this is the pairwise stage's Needleman-Wunsch.  A row holds
+ * the first unpadded() codes of its sequence (src/sequence.hpp:110-119: the
+ * padded length less the padding codes at positions 1 and 2 of the last
+ * block); the result gives the column of each of those residues, ascending,
+ * as the reference's gapped sequence keeps them (src/pgalign/sequence.cuh,
+ * m_columns).  Sequences holding the gap code 0x19 are refused
+ * (MSA_ERR_GAP_CODE). */
+
+/* Total residues of the rows (the size of msa_pg_align's columns). */
+int msa_pg_residues(const uint16_t* blocks, const int64_t* offset, const int32_t* size, int32_t nseq,
+                    int64_t* residues);
+/* nodes: the guide tree as msa_nj_build gives it (2n-1 nodes, join t = node
+ * n+t with children below it).  columns receives, sequence by sequence in
+ * database order, the column of every residue; length the number of columns;
+ * join_scores (may be NULL) the optimal sum-of-pairs score of each join
+ * (n-1, in node order).  Synchronous, on the current device. */
+int msa_pg_align(const uint16_t* blocks, const int64_t* offset, const int32_t* size, int32_t nseq,
+                 const msa_nj_node* nodes, const int32_t* table625, int32_t penalty,
+                 int32_t* columns, int64_t* length, int64_t* join_scores);
+
 /* ---- FASTA loading -------------------------------------------------------------
  * Replaces, on the device, the reference's FASTA loader and its packing for the
  * pairwise stage: io::parser::fasta (src/io/loader/parser/fasta.cpp:26-70),

@@ -131,6 +131,8 @@ SIGNATURES = {
     "msa_nw_tune": (ctypes.c_int, [_vp, _vp, _vp, _i32, _vp, _i32, _i64, ctypes.c_char_p, _vp, ctypes.c_size_t]),
     "msa_nj_build": (ctypes.c_int, [_vp, _i32, _i32, _vp]),
     "msa_nj_build_device": (ctypes.c_int, [_vp, _i32, _i32, _vp, _vp]),
+    "msa_pg_residues": (ctypes.c_int, [_vp, _vp, _vp, _i32, ctypes.POINTER(_i64)]),
+    "msa_pg_align": (ctypes.c_int, [_vp, _vp, _vp, _i32, _vp, _vp, _i32, _vp, ctypes.POINTER(_i64), _vp]),
     "msa_fasta_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
     "msa_fasta_destroy": (ctypes.c_int, [_vp]),
     "msa_fasta_parse_device": (ctypes.c_int, [_vp, _vp, _i64, _vp]),
@@ -335,6 +337,34 @@ def nj_build(distances, n, variant=NJ_SYMMETRIC):
     nodes = np.zeros(max(2 * n - 1, 0) if n >= 2 else 0, dtype=NJ_NODE)
     check(lib().msa_nj_build(_ptr(d), int(n), int(variant), _ptr(nodes)))
     return nodes
+
+
+def pg_residues(blocks, offsets, sizes):
+    """msa_pg_residues: residues the rows of msa_pg_align hold (unpadded lengths)."""
+    blocks = np.ascontiguousarray(blocks, dtype=np.uint16)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+    total = _i64(0)
+    check(lib().msa_pg_residues(_ptr(blocks), _ptr(offsets), _ptr(sizes), len(sizes), ctypes.byref(total)))
+    return total.value
+
+
+def pg_align(blocks, offsets, sizes, nodes, table625, penalty):
+    """msa_pg_align: the progressive multiple alignment along a guide tree
+    (NJ_NODE records, 2n-1).  Returns (columns int32 per residue in database
+    order, alignment length, int64 optimal sum-of-pairs score of each join)."""
+    blocks = np.ascontiguousarray(blocks, dtype=np.uint16)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+    nodes = np.ascontiguousarray(nodes, dtype=NJ_NODE)
+    table625 = np.ascontiguousarray(np.asarray(table625).reshape(625), dtype=np.int32)
+    n = len(sizes)
+    cols = np.zeros(max(pg_residues(blocks, offsets, sizes), 1), dtype=np.int32)
+    scores = np.zeros(max(n - 1, 1), dtype=np.int64)
+    length = _i64(0)
+    check(lib().msa_pg_align(_ptr(blocks), _ptr(offsets), _ptr(sizes), n, _ptr(nodes), table625.ctypes.data,
+                             int(penalty), cols.ctypes.data, ctypes.byref(length), scores.ctypes.data))
+    return cols[:pg_residues(blocks, offsets, sizes)], length.value, scores[:max(n - 1, 0)]
 
 
 def nj_build_device(d_scores, n, variant, d_nodes, stream=None):

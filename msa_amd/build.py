@@ -23,7 +23,8 @@ ARCH = os.environ.get("MSA_OFFLOAD_ARCH", "gfx950")
 
 # Longest compiles first (the batched kernel's three tile configurations).
 HIP_SOURCES = ["nw_batched.hip", "nw_batched_32x16.hip", "nw_batched_16x16.hip", "runtime.hip", "nw_plain.hip",
-               "nw_long.hip", "nw_plan.hip", "nw_noalloc.hip", "nw_serve.hip", "nw_tune.hip", "nj.hip", "fasta.hip"]
+               "nw_long.hip", "nw_plan.hip", "nw_noalloc.hip", "nw_serve.hip", "nw_tune.hip", "nj.hip", "pgalign.hip",
+               "fasta.hip"]
 HIP_FLAGS = ["-std=c++17", "-O3", "-fPIC", "--offload-arch=" + ARCH,
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-command-line-argument"]

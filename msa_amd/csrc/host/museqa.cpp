@@ -543,6 +543,170 @@ namespace museqa
         }
     }
 
+    // ---- pgalign ----------------------------------------------------------
+
+    namespace pgalign
+    {
+        sequence::sequence(const museqa::sequence& original) : m_original {original}
+        {
+            const size_t n = original.unpadded();
+            m_columns.resize(n + 1);
+            for (size_t i = 0; i <= n; ++i) m_columns[i] = uint32_t(i);
+        }
+
+        sequence::sequence(const museqa::sequence& original, std::vector<uint32_t> columns, size_t length)
+            : m_original {original}, m_columns {std::move(columns)}
+        {
+            if (m_columns.size() != original.unpadded()) throw exception("one column per residue is needed");
+            for (size_t i = 0; i < m_columns.size(); ++i)
+                if (m_columns[i] >= length || (i && m_columns[i] <= m_columns[i - 1]))
+                    throw exception("residue columns must ascend within the alignment's length");
+            m_columns.push_back(uint32_t(length));
+        }
+
+        // src/pgalign/sequence.cu:24-50: a binary search over the residues'
+        // columns.
+        encoder::unit sequence::operator[](ptrdiff_t column) const
+        {
+            if (column < 0 || size_t(column) >= length()) return encoder::end;
+            const auto first = m_columns.begin(), last = m_columns.end() - 1;
+            const auto it = std::lower_bound(first, last, uint32_t(column));
+            return (it != last && *it == uint32_t(column)) ? m_original[it - first] : encoder::gap;
+        }
+
+        std::string sequence::decode() const
+        {
+            std::string out(length(), encoder::decode(encoder::gap));
+            for (size_t i = 0; i + 1 < m_columns.size(); ++i) out[m_columns[i]] = encoder::decode(m_original[i]);
+            return out;
+        }
+
+        const sequence& alignment::operator[](ptrdiff_t row) const
+        {
+            if (row < 0 || size_t(row) >= m_rows.size()) throw exception("alignment row out of range");
+            return m_rows[row];
+        }
+
+        // src/pgalign/alignment.cuh:58-65: the rows [displ, displ + size).
+        alignment alignment::slice(ptrdiff_t displ, size_t size) const
+        {
+            if (displ < 0 || size_t(displ) + size > m_rows.size()) throw exception("alignment slice out of range");
+            return alignment {std::vector<sequence>(m_rows.begin() + displ, m_rows.begin() + displ + size)};
+        }
+
+        namespace
+        {
+            // Progressive profile alignment along the guide tree in the HIP
+            // kernels behind msa_pg_align (every join optimal under the
+            // sum-of-pairs score with the table's linear gap).
+            struct hip_progressive : public algorithm
+            {
+                alignment run(const context& ctx) const override
+                {
+                    const size_t n = ctx.count;
+                    if (n > ctx.db.count()) throw exception("the database holds fewer sequences than requested");
+                    if (n == 0) return alignment {};
+                    if (ctx.tree.leaves() != n || ctx.tree.count() != 2 * n - 1)
+                        throw exception("the guide tree does not join the requested sequences");
+                    const pairwise::scoring_table table =
+                        ctx.table ? *ctx.table : pairwise::scoring_table::make("default");
+                    int32_t t625[625];
+                    int32_t penalty = 0;
+                    pairwise::integer_table(table, t625, penalty);
+                    std::vector<uint16_t> blocks;
+                    std::vector<int64_t> offset(n);
+                    std::vector<int32_t> size(n);
+                    std::vector<size_t> at(n + 1, 0);
+                    for (size_t s = 0; s < n; ++s) {
+                        const auto& seq = ctx.db[ptrdiff_t(s)].contents;
+                        offset[s] = int64_t(blocks.size());
+                        size[s] = int32_t(seq.size());
+                        blocks.insert(blocks.end(), seq.blocks().begin(), seq.blocks().end());
+                        at[s + 1] = at[s] + seq.unpadded();
+                    }
+                    std::vector<int32_t> columns(std::max<size_t>(at[n], 1));
+                    int64_t length = 0;
+                    if (n > 1) {
+                        int devices = 0;
+                        check(msa_device_count(&devices));
+                        if (devices < 1) throw exception("no GPU available for the profile aligner");
+                    }
+                    if (n == 1) {
+                        for (size_t i = 0; i < at[1]; ++i) columns[i] = int32_t(i);
+                        length = int64_t(at[1]);
+                    } else {
+                        check(msa_pg_align(blocks.data(), offset.data(), size.data(), int32_t(n),
+                                           reinterpret_cast<const msa_nj_node*>(ctx.tree.raw().data()), t625, penalty,
+                                           columns.data(), &length, nullptr));
+                    }
+                    std::vector<sequence> rows;
+                    rows.reserve(n);
+                    for (size_t s = 0; s < n; ++s)
+                        rows.emplace_back(ctx.db[ptrdiff_t(s)].contents,
+                                          std::vector<uint32_t>(columns.begin() + at[s], columns.begin() + at[s + 1]),
+                                          size_t(length));
+                    return alignment {std::move(rows)};
+                }
+            };
+
+            using factory = std::function<std::unique_ptr<algorithm>()>;
+
+            std::unique_ptr<algorithm> make_progressive() { return std::unique_ptr<algorithm>(new hip_progressive()); }
+
+            // src/pgalign/pgalign.cpp:24-28: every name of the reference runs
+            // the GPU progressive aligner (the reference's "myers" is a stub).
+            const std::vector<std::pair<std::string, factory>>& registry()
+            {
+                static const std::vector<std::pair<std::string, factory>> r = {
+                    {"default", make_progressive},
+                    {"myers", make_progressive},
+                    {"sequential", make_progressive},
+                    {"myers-sequential", make_progressive},
+                    {"hip", make_progressive},
+                };
+                return r;
+            }
+        }
+
+        bool algorithm::has(const std::string& name)
+        {
+            for (const auto& e : registry())
+                if (e.first == name) return true;
+            return false;
+        }
+
+        std::unique_ptr<algorithm> algorithm::make(const std::string& name)
+        {
+            for (const auto& e : registry())
+                if (e.first == name) return e.second();
+            raise("unknown pgalign algorithm '%s'", name);
+        }
+
+        const std::vector<std::string>& algorithm::list() noexcept
+        {
+            static const std::vector<std::string> names = [] {
+                std::vector<std::string> v;
+                for (const auto& e : registry()) v.push_back(e.first);
+                return v;
+            }();
+            return names;
+        }
+
+        alignment run(const museqa::database& db, const phylogeny::guidetree& tree, size_t count,
+                      const std::string& name)
+        {
+            const auto worker = algorithm::make(name);
+            return worker->run({db, tree, count, nullptr});
+        }
+
+        alignment run(const museqa::database& db, const phylogeny::guidetree& tree, size_t count,
+                      const pairwise::scoring_table& table, const std::string& name)
+        {
+            const auto worker = algorithm::make(name);
+            return worker->run({db, tree, count, &table});
+        }
+    }
+
     // ---- module::pairwise (src/pairwise.cpp:27-55) ------------------------
 
     namespace
@@ -583,5 +747,24 @@ namespace museqa
     {
         auto tree = museqa::phylogeny::run(previous.distances, previous.count, option(opts, "phylogeny", "default"));
         return conduit {previous.db, std::move(tree)};
+    }
+
+    // ---- module::pgalign (src/pgalign.cpp:25-47) --------------------------
+
+    bool module::pgalign::check(const options& opts)
+    {
+        const auto algoname = option(opts, "pgalign", "default");
+        if (!museqa::pgalign::algorithm::has(algoname)) raise("unknown pgalign algorithm chosen: '%s'", algoname);
+        const auto tablename = option(opts, "scoring-table", "default");
+        if (!museqa::pairwise::scoring_table::has(tablename)) raise("unknown scoring table chosen: '%s'", tablename);
+        return true;
+    }
+
+    module::pgalign::conduit module::pgalign::run(const options& opts, const phylogeny::conduit& previous)
+    {
+        const auto table = museqa::pairwise::scoring_table::make(option(opts, "scoring-table", "default"));
+        auto aligned = museqa::pgalign::run(previous.db, previous.tree, previous.tree.leaves(), table,
+                                            option(opts, "pgalign", "default"));
+        return conduit {std::move(aligned)};
     }
 }

@@ -273,6 +273,80 @@ namespace museqa
                       const std::string& algorithm = "default");
     }
 
+    namespace pgalign
+    {
+        // A sequence with gaps (src/pgalign/sequence.cuh): the sequence's
+        // first unpadded() residues and, per residue, its column, ascending;
+        // the last entry of columns() is the row's length, gaps included.
+        class sequence
+        {
+            public:
+                sequence() = default;
+                // The sequence alone: residue i in column i.
+                explicit sequence(const museqa::sequence& original);
+                // The sequence placed in an alignment: one column per residue
+                // and the alignment's length.
+                sequence(const museqa::sequence& original, std::vector<uint32_t> columns, size_t length);
+
+                // Code at a column: the residue's code, encoder::gap between
+                // residues, encoder::end past the row's length
+                // (src/pgalign/sequence.cu:24-50).
+                encoder::unit operator[](ptrdiff_t column) const;
+                size_t length() const noexcept { return m_columns.empty() ? 0 : m_columns.back(); }
+                size_t residues() const noexcept { return m_columns.empty() ? 0 : m_columns.size() - 1; }
+                std::string decode() const;
+                const std::vector<uint32_t>& columns() const noexcept { return m_columns; }
+                const museqa::sequence& original() const noexcept { return m_original; }
+
+            private:
+                museqa::sequence m_original;
+                std::vector<uint32_t> m_columns;
+        };
+
+        // The alignment of one or more sequences (src/pgalign/alignment.cuh):
+        // one gapped row per sequence, in database order.
+        class alignment
+        {
+            public:
+                alignment() = default;
+                explicit alignment(std::vector<sequence> rows) : m_rows {std::move(rows)} {}
+
+                const sequence& operator[](ptrdiff_t row) const;
+                size_t count() const noexcept { return m_rows.size(); }
+                size_t length() const noexcept { return m_rows.empty() ? 0 : m_rows.front().length(); }
+                alignment slice(ptrdiff_t displ, size_t size) const;
+
+            private:
+                std::vector<sequence> m_rows;
+        };
+
+        // src/pgalign/pgalign.cuh:22-31, plus the scoring table the joins
+        // use (the reference's stub needs none; null = the "default" table).
+        struct context
+        {
+            const museqa::database& db;
+            const phylogeny::guidetree& tree;
+            const size_t count;
+            const pairwise::scoring_table* table = nullptr;
+        };
+
+        struct algorithm
+        {
+            virtual ~algorithm() = default;
+            virtual alignment run(const context&) const = 0;
+
+            static bool has(const std::string&);
+            static std::unique_ptr<algorithm> make(const std::string&);
+            static const std::vector<std::string>& list() noexcept;
+        };
+
+        // src/pgalign/pgalign.cuh:82-90.
+        alignment run(const museqa::database& db, const phylogeny::guidetree& tree, size_t count,
+                      const std::string& algorithm = "default");
+        alignment run(const museqa::database& db, const phylogeny::guidetree& tree, size_t count,
+                      const pairwise::scoring_table& table, const std::string& algorithm = "default");
+    }
+
     namespace module
     {
         // The pipeline module contract: option values in, conduit out.
@@ -304,6 +378,22 @@ namespace museqa
             static const char* name() { return "phylogeny"; }
             static bool check(const options& opts);
             static conduit run(const options& opts, const pairwise::conduit& previous);
+        };
+
+        // src/pgalign.cpp:25-47: the phylogeny conduit in, the alignment out;
+        // the joins score with the option "scoring-table" (as the pairwise
+        // module reads it).
+        struct pgalign
+        {
+            struct conduit
+            {
+                museqa::pgalign::alignment aligned;
+            };
+
+            using options = std::map<std::string, std::string>;
+            static const char* name() { return "pgalign"; }
+            static bool check(const options& opts);
+            static conduit run(const options& opts, const phylogeny::conduit& previous);
         };
     }
 }

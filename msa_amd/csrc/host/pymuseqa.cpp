@@ -120,4 +120,31 @@ PYBIND11_MODULE(_museqa, m)
         auto c = module::phylogeny::run(opts, module::pairwise::conduit {db, d, count});
         return py::make_tuple(c.db, c.tree);
     });
+
+    // museqa::pgalign (src/pgalign/pgalign.cuh, alignment.cuh, sequence.cuh);
+    // the reference ships no Python binding for it either.
+    py::class_<pgalign::alignment>(m, "Alignment")
+        .def_property_readonly("count", &pgalign::alignment::count)
+        .def_property_readonly("length", &pgalign::alignment::length)
+        .def("row", [](const pgalign::alignment& a, ptrdiff_t i) {
+            const auto& r = a[i];
+            const std::vector<uint32_t> cols(r.columns().begin(), r.columns().end() - 1);
+            return py::make_tuple(cols, r.decode());
+        })
+        .def("at", [](const pgalign::alignment& a, ptrdiff_t i, ptrdiff_t column) {
+            return std::string(1, encoder::decode(a[i][column]));
+        })
+        .def("slice", &pgalign::alignment::slice);
+    m.def("pgalign_run", [](const database& db, const phylogeny::guidetree& tree, size_t count,
+                            const pairwise::scoring_table& table, const std::string& algorithm) {
+        py::gil_scoped_release release;
+        return pgalign::run(db, tree, count, table, algorithm);
+    });
+    m.def("pgalign_algorithms", &pgalign::algorithm::list);
+    m.def("pgalign_has_algorithm", &pgalign::algorithm::has);
+    m.def("pgalign_module_check", &module::pgalign::check);
+    m.def("pgalign_module_run", [](const module::pgalign::options& opts, const database& db,
+                                   const phylogeny::guidetree& tree) {
+        return module::pgalign::run(opts, module::phylogeny::conduit {db, tree}).aligned;
+    });
 }

@@ -302,6 +302,49 @@ class NjOracle:
         return nodes
 
 
+class PgOracle:
+    """ctypes wrapper of pg_oracle.c (in oracle/_build/liboracle.so): the
+    progressive profile alignment along a guide tree."""
+
+    def __init__(self, path=None):
+        path = path or os.path.join(HERE, "_build", "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle library not built: run `make -C oracle`")
+        self.lib = ctypes.CDLL(path)
+        self.lib.pg_oracle_planted.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                               ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int,
+                                               ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.c_void_p]
+        self.lib.pg_oracle_planted.restype = ctypes.c_int
+        self.lib.pg_unpadded.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+        self.lib.pg_unpadded.restype = ctypes.c_int64
+
+    def unpadded(self, blocks, offsets, sizes):
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint16)
+        return np.array([self.lib.pg_unpadded(blocks[int(o):].ctypes.data if int(s) else None, int(s))
+                         for o, s in zip(offsets, sizes)], dtype=np.int64)
+
+    def align(self, blocks, offsets, sizes, nodes, table625, penalty, mistake=0):
+        """(columns per residue in database order, length, join scores).  A
+        non-zero `mistake` plants an error (tests of the pins only)."""
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint16)
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+        nodes = np.ascontiguousarray(nodes, dtype=NJ_NODE)
+        table = np.ascontiguousarray(np.asarray(table625).reshape(625), dtype=np.int32)
+        n = len(sizes)
+        total = int(self.unpadded(blocks, offsets, sizes).sum()) if n else 0
+        cols = np.zeros(max(total, 1), dtype=np.int32)
+        scores = np.zeros(max(n - 1, 1), dtype=np.int64)
+        length = ctypes.c_int64(0)
+        rc = self.lib.pg_oracle_planted(blocks.ctypes.data if len(blocks) else None, offsets.ctypes.data,
+                                        sizes.ctypes.data, n, nodes.ctypes.data if len(nodes) else None,
+                                        table.ctypes.data, int(penalty), int(mistake), cols.ctypes.data,
+                                        ctypes.byref(length), scores.ctypes.data)
+        if rc != 0:
+            raise ValueError("invalid guide tree or gap code")
+        return cols[:total], length.value, scores[:max(n - 1, 0)]
+
+
 def tree_splits(children, n):
     """Bipartitions of the leaves 0..n-1 of a tree, each as the frozenset of
     the side that does not hold leaf 0; `children` maps an internal node to
