@@ -22,6 +22,7 @@ import argparse
 import ctypes
 import json
 import os
+import re
 import sys
 import time
 
@@ -50,6 +51,8 @@ def make_workload(name, tables):
         return W.uniform_small(tables)
     if name == "dna":
         return W.long_dna()
+    if re.fullmatch(r"case[1-6]", name):
+        return W.reference_case(int(name[4:]), tables)
     raise SystemExit("unknown workload %r" % name)
 
 
@@ -281,7 +284,7 @@ def serving_run(N, tables, rates, seconds, pool_size=48, slots=4, seed=5, log_fn
     return out
 
 
-def measure(N, w, steps, warmup, plain_steps, oracle_pairs, stream, out, tuning):
+def measure(N, w, steps, warmup, plain_steps, oracle_pairs, stream, out, tuning, oracle_cells=2e10):
     """The auto plan (timed as hipGraph replays) and the plain kernel on one
     workload, both outputs compared on every pair, plus an oracle sample."""
     cells = w.cells()
@@ -290,7 +293,7 @@ def measure(N, w, steps, warmup, plain_steps, oracle_pairs, stream, out, tuning)
     plain = run_plan(N, w, N.KERNEL_PLAIN, plain_steps, 1, stream, out)
     log("plain:     %.2f ms/step, %.1f GCUPS" % (plain["ms"], cells / plain["ms"] / 1e6))
     same = bool(np.array_equal(batched["scores"], plain["scores"]))
-    oracle_ok, oracle_n = oracle_sample(w, batched["scores"], oracle_pairs)
+    oracle_ok, oracle_n = oracle_sample(w, batched["scores"], oracle_pairs, max_cells=oracle_cells)
     log("auto == plain on all pairs:", same, "| oracle sample of", oracle_n, "pairs:", oracle_ok)
     return batched, plain, same, oracle_ok, oracle_n
 
@@ -484,6 +487,8 @@ def main():
                     help="rates (jobs/s, comma separated) of the serving figures in the main line; '' = none")
     ap.add_argument("--no-other-configs", dest="other_configs", action="store_false",
                     help="skip the uniform-512 and dna-64x30000 figures of the default line")
+    ap.add_argument("--no-reference-cases", dest="reference_cases", action="store_false",
+                    help="skip the figures over the reference's db/case1..6.fasta")
     ap.add_argument("--phylogeny", action="store_true",
                     help="only the neighbor-joining stage over the workload's scores (its own JSON line)")
     ap.add_argument("--fasta", action="store_true",
@@ -597,6 +602,19 @@ def main():
             o_out.close()
             rec = config_record(wo, b, pl, eq, ok_o, n_o)
             result["other_configs"][wo.name] = rec
+            ok_others = ok_others and eq and ok_o
+    # The reference's own benchmark databases (db/case1..6.fasta, real viral
+    # genomes), parsed by the GPU FASTA loader and timed the same way.
+    if args.workload == "uneven" and args.reference_cases:
+        result["reference_cases"] = {}
+        for k in range(1, 7):
+            wo = make_workload("case%d" % k, tables)
+            log("workload:", wo.description, "pairs", wo.npairs, "cells %.4e" % wo.cells())
+            o_out = N.DeviceBuffer(4 * wo.npairs)
+            b, pl, eq, ok_o, n_o = measure(N, wo, min(args.steps, 3), 1, 1, 24, stream, o_out, tuning,
+                                           oracle_cells=4e9)
+            o_out.close()
+            result["reference_cases"][wo.name] = config_record(wo, b, pl, eq, ok_o, n_o)
             ok_others = ok_others and eq and ok_o
     if args.serve_rate:
         rates = [float(r) for r in str(args.serve_rate).split(",")]

@@ -12,6 +12,8 @@ R4 N5 D6 Q7 E8 H9 I10 L11 K12 M13 F14 P15 S16 W17 Y18 V19 ...
 Sequences are packed three codes per uint16 block at bit shifts 1, 6 and 11,
 with bit 0 set on the block that holds padding (src/encoder.cpp:76-98).
 """
+import os
+
 import numpy as np
 
 MASK64 = (1 << 64) - 1
@@ -257,3 +259,77 @@ def fasta_database(seed=6, target_bytes=512 << 20, block_bytes=64 << 20, width=6
     desc = ("protein FASTA, %d x a %.1f MiB block of records with L=clip(round(lognormal(ln400,1)),50,5000), "
             "BLOSUM62 background, %d residues per line, seed %d" % (reps, len(block) / 2**20, width, seed))
     return text, desc
+
+
+# ---- the reference's own benchmark databases -----------------------------------
+
+_DB = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests", "golden", "db")
+
+REFERENCE_CASES = {
+    1: "50 complete viral genomes of about 10,000 nt",
+    2: "100 complete viral genomes of about 10,000 nt",
+    3: "200 complete viral genomes of about 5,000 nt",
+    4: "400 complete viral genomes of about 5,000 nt",
+    5: "400 complete viral genomes of about 2,000 nt",
+    6: "800 complete viral genomes of about 2,000 nt",
+}
+
+
+def reference_case_text(k):
+    """The text of the reference's db/case<k>.fasta (kept xz-compressed under
+    tests/golden/db/, checked against the original's sha256)."""
+    import hashlib
+    import json
+    import lzma
+    name = "case%d.fasta" % k
+    with open(os.path.join(_DB, name + ".xz"), "rb") as f:
+        text = lzma.decompress(f.read())
+    with open(os.path.join(_DB, "digests.json")) as f:
+        want = json.load(f)[name]["sha256"]
+    if hashlib.sha256(text).hexdigest() != want:
+        raise RuntimeError("%s does not match the reference's file" % name)
+    return text
+
+
+class StoredWorkload(Workload):
+    """A workload whose sequences come as a packed store (blocks, offsets and
+    sizes), e.g. from the FASTA loader; lengths are the residues before the
+    padding of the last block."""
+
+    def __init__(self, name, blocks, offsets, sizes, table, penalty, description):
+        self.name = name
+        self.table = np.asarray(table, dtype=np.int32).reshape(25, 25)
+        self.penalty = int(penalty)
+        self.description = description
+        self.blocks = np.ascontiguousarray(blocks, dtype=np.uint16)
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        self.sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+        lengths = []
+        for o, s in zip(self.offsets, self.sizes):
+            codes = unpack(self.blocks, int(o), int(s))
+            n = len(codes)
+            while n > len(codes) - 2 and n > 0 and codes[n - 1] == 24:
+                n -= 1
+            lengths.append(n)
+        self.lengths = np.array(lengths, dtype=np.int64)
+        self.codes = None
+
+    @property
+    def nseq(self):
+        return len(self.sizes)
+
+
+def reference_case(k, tables, table="default"):
+    """db/case<k>.fasta parsed on the GPU (msa_fasta_parse), scored with a
+    named table (the reference's command line defaults to "default")."""
+    from . import _native as N
+    text = reference_case_text(k)
+    loader = N.FastaLoader()
+    try:
+        loader.parse(text)
+        blocks, offsets, sizes, _, _ = loader.store()
+    finally:
+        loader.close()
+    t, p = tables[table]
+    return StoredWorkload("case%d" % k, blocks, offsets, sizes, t, p,
+                          "reference db/case%d.fasta: %s, %s p=%d" % (k, REFERENCE_CASES[k], table, p))

@@ -80,3 +80,24 @@ def test_structure_of_a_synthetic_file():
     assert list(offset) == list(np.concatenate([[0], np.cumsum(size)[:-1]]))
     for s, got in zip(seqs, H.decode_store(blocks, offset, size)):
         assert got[:len(s)] == s.decode() and set(got[len(s):]) <= {"*"}
+
+
+@pytest.mark.parametrize("k", range(1, 7))
+def test_reference_benchmark_databases(k, tmp_path):
+    """db/case<k>.fasta (kept compressed under tests/golden/db/, sha256 of the
+    original checked on load): the oracle's store has one sequence per '>'
+    record, each of the residues its lines hold, and equals the reference's
+    own parser where that is built."""
+    from msa_amd import workloads as W
+    text = W.reference_case_text(k)
+    records = []
+    for line in text.split(b"\n"):
+        if line.startswith(b">"):
+            records.append(0)
+        elif records:
+            records[-1] += len(line.strip())
+    blocks, offset, size, _, _ = ORACLE.parse(text)
+    assert len(size) == len(records)
+    assert np.array_equal(size, [(r + 2) // 3 for r in records])
+    if M.ReferenceFasta.available():
+        assert_same_as_reference(text, tmp_path)

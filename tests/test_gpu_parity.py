@@ -551,3 +551,28 @@ def test_tuner_writes_a_table_that_plans_read(tmp_path):
     with open(path) as f:
         assert json.load(f)["bins"] == table["bins"]
     assert_oracle(db, T, p, kernel="auto", tuning_file=path)
+
+
+@pytest.mark.parametrize("case", [1, 2, 3, 4, 5, 6])
+def test_reference_benchmark_databases(case):
+    """The reference's own benchmark inputs (db/case1..6.fasta: 50-800 viral
+    genomes of 2k-10k nt with N and ambiguity letters, 70-column lines and an
+    empty line after each record), parsed by the GPU FASTA loader into exactly
+    the oracle parser's store, then scored all-pairs with the reference's
+    default table: the auto plan (long, batched and plain routing) equals the
+    plain kernel on every pair and the oracle on a sample."""
+    from oracle import museqa_ref as M
+    tables = H.tables()
+    w = W.reference_case(case, tables)
+    exp_store = M.FastaOracle().parse(W.reference_case_text(case))
+    for got, exp in zip((w.blocks, w.offsets, w.sizes), exp_store[:3]):
+        assert np.array_equal(got, exp)
+    db = (w.blocks, w.offsets, w.sizes)
+    auto = gpu_scores(db, w.table, w.penalty, use_graph=True)
+    plain = gpu_scores(db, w.table, w.penalty, kernel="plain")
+    assert np.array_equal(auto, plain)
+    rng = np.random.default_rng(case)
+    idx = rng.choice(w.npairs, 6, replace=False)
+    pairs = np.array([divmod_pair(int(c)) for c in idx], dtype=np.int32)
+    exp = H.oracle_scores(db, w.table, w.penalty, pairs=pairs, workers=6)
+    assert np.array_equal(auto[idx].astype(np.float32), exp)
