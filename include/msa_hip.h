@@ -373,22 +373,28 @@ typedef struct {
     size_t device_bytes;        /* HBM arena for admitted jobs (0 = 1 GiB) */
     size_t host_bytes;          /* pinned staging arena (0 = 256 MiB) */
     msa_nw_options options;     /* routing for every job */
+    size_t store_bytes;         /* HBM of the resident sequence store (0 = 256 MiB; at most 4 GiB - 256) */
 } msa_srv_config;
 
 typedef struct {
     double arrival_us;          /* microseconds since msa_srv_open */
     double admitted_us;
-    double finished_us;         /* scores on the host */
+    double finished_us;         /* scores complete in the device arena */
     int64_t pairs;
     int64_t cells;
-    int64_t device_bytes;       /* arena bytes the job held */
+    int64_t device_bytes;       /* arena bytes the job held (data block + scores) */
     int32_t slot;
+    int64_t upload_bytes;       /* host->device bytes of the job (its data block) */
+    int64_t sequence_bytes;     /* of which sequence blocks: 0 for a job over the resident store */
+    double fetch_us;            /* the score copy inside msa_srv_fetch (0 for msa_srv_fetch_device) */
 } msa_srv_job_stats;
 
 typedef struct {
     int64_t submitted, completed, running, pending;
     int64_t cells;              /* cells of completed jobs */
     int64_t device_bytes_used, device_bytes_capacity;
+    int64_t store_bytes_used, store_bytes_capacity;
+    int64_t store_sequences;    /* live ids in the resident store */
 } msa_srv_stats;
 
 int msa_srv_open(msa_server* srv, const msa_srv_config* cfg);
@@ -399,8 +405,27 @@ int msa_srv_submit(msa_server srv, const uint16_t* blocks, const int64_t* offset
 int msa_srv_poll(msa_server srv, uint64_t job_id, int* done);
 /* Waits up to timeout_ms (negative: forever); MSA_ERR_NOT_READY on timeout. */
 int msa_srv_wait(msa_server srv, uint64_t job_id, int32_t timeout_ms);
-/* Copies the scores (pair order) of a finished job, its timing, and forgets it. */
+/* Resident sequence store.  msa_srv_store_add copies nseq sequences (the
+ * store layout of msa_nw_*) into the server's device store and returns once
+ * they are there; they get the ids *first_id .. *first_id + nseq - 1 and stay
+ * until msa_srv_store_remove(first_id) drops the whole call's set
+ * (MSA_ERR_BUSY while a queued or running job reads it).  MSA_ERR_OUT_OF_MEMORY
+ * when the store is full. */
+int msa_srv_store_add(msa_server srv, const uint16_t* blocks, const int64_t* offset, const int32_t* size,
+                      int32_t nseq, int64_t* first_id);
+int msa_srv_store_remove(msa_server srv, int64_t first_id);
+/* An all-pairs job over resident sequences, in the order of `ids` (pairs and
+ * scores as msa_srv_submit over those sequences): only its work lists travel
+ * to the device. */
+int msa_srv_submit_resident(msa_server srv, const int64_t* ids, int32_t nseq, uint64_t* job_id);
+/* A finished job's scores stay in the device arena (and count against it)
+ * until fetched.  msa_srv_fetch copies them (pair order) to host memory,
+ * returns the job's timing, and forgets the job. */
 int msa_srv_fetch(msa_server srv, uint64_t job_id, int32_t* out, int64_t capacity, msa_srv_job_stats* stats);
+/* The same into device memory, as a copy queued on `stream` (not a capturing
+ * one); the arena block is reused once that copy has run. */
+int msa_srv_fetch_device(msa_server srv, uint64_t job_id, int32_t* d_out, int64_t capacity, msa_stream stream,
+                         msa_srv_job_stats* stats);
 int msa_srv_stats_get(msa_server srv, msa_srv_stats* stats);
 /* Finishes the running jobs, drops the queued ones, frees everything. */
 int msa_srv_close(msa_server srv);

@@ -63,13 +63,15 @@ class PlanInfo(ctypes.Structure):
 
 class SrvConfig(ctypes.Structure):
     _fields_ = [("table625", _vp), ("penalty", _i32), ("slots", _i32), ("max_length", _i32),
-                ("max_sequences", _i32), ("device_bytes", _sz), ("host_bytes", _sz), ("options", NwOptions)]
+                ("max_sequences", _i32), ("device_bytes", _sz), ("host_bytes", _sz), ("options", NwOptions),
+                ("store_bytes", _sz)]
 
 
 class SrvJobStats(ctypes.Structure):
     _fields_ = [("arrival_us", ctypes.c_double), ("admitted_us", ctypes.c_double),
                 ("finished_us", ctypes.c_double), ("pairs", _i64), ("cells", _i64),
-                ("device_bytes", _i64), ("slot", _i32)]
+                ("device_bytes", _i64), ("slot", _i32), ("upload_bytes", _i64), ("sequence_bytes", _i64),
+                ("fetch_us", ctypes.c_double)]
 
 
 class FastaInfo(ctypes.Structure):
@@ -83,7 +85,8 @@ class FastaTiming(ctypes.Structure):
 
 class SrvStats(ctypes.Structure):
     _fields_ = [("submitted", _i64), ("completed", _i64), ("running", _i64), ("pending", _i64),
-                ("cells", _i64), ("device_bytes_used", _i64), ("device_bytes_capacity", _i64)]
+                ("cells", _i64), ("device_bytes_used", _i64), ("device_bytes_capacity", _i64),
+                ("store_bytes_used", _i64), ("store_bytes_capacity", _i64), ("store_sequences", _i64)]
 
 
 # name -> (restype, argtypes); every symbol include/msa_hip.h declares.
@@ -146,7 +149,11 @@ SIGNATURES = {
     "msa_srv_submit": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, ctypes.POINTER(ctypes.c_uint64)]),
     "msa_srv_poll": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]),
     "msa_srv_wait": (ctypes.c_int, [_vp, ctypes.c_uint64, _i32]),
+    "msa_srv_store_add": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, ctypes.POINTER(_i64)]),
+    "msa_srv_store_remove": (ctypes.c_int, [_vp, _i64]),
+    "msa_srv_submit_resident": (ctypes.c_int, [_vp, _vp, _i32, ctypes.POINTER(ctypes.c_uint64)]),
     "msa_srv_fetch": (ctypes.c_int, [_vp, ctypes.c_uint64, _vp, _i64, ctypes.POINTER(SrvJobStats)]),
+    "msa_srv_fetch_device": (ctypes.c_int, [_vp, ctypes.c_uint64, _vp, _i64, _vp, ctypes.POINTER(SrvJobStats)]),
     "msa_srv_stats_get": (ctypes.c_int, [_vp, ctypes.POINTER(SrvStats)]),
     "msa_srv_close": (ctypes.c_int, [_vp]),
 }
@@ -376,10 +383,10 @@ class Server:
     """msa_srv_*: a serving loop that admits all-pairs jobs as they arrive."""
 
     def __init__(self, table625, penalty, slots=4, max_length=2048, max_sequences=4096,
-                 device_bytes=1 << 30, host_bytes=256 << 20, opts=None):
+                 device_bytes=1 << 30, host_bytes=256 << 20, opts=None, store_bytes=256 << 20):
         self._table = np.ascontiguousarray(np.asarray(table625).reshape(625), dtype=np.int32)
         cfg = SrvConfig(self._table.ctypes.data, int(penalty), int(slots), int(max_length), int(max_sequences),
-                        int(device_bytes), int(host_bytes), opts or options())
+                        int(device_bytes), int(host_bytes), opts or options(), int(store_bytes))
         h = _vp()
         check(lib().msa_srv_open(ctypes.byref(h), ctypes.byref(cfg)))
         self.handle = h.value
@@ -395,6 +402,39 @@ class Server:
         n = len(sizes)
         self.npairs[job.value] = n * (n - 1) // 2
         return job.value
+
+    def store_add(self, blocks, offsets, sizes):
+        """Puts sequences into the resident store; returns their first id
+        (the ids are consecutive, in input order)."""
+        blocks = np.ascontiguousarray(blocks, dtype=np.uint16)
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+        first = _i64(0)
+        check(lib().msa_srv_store_add(self.handle, _ptr(blocks), _ptr(offsets), _ptr(sizes), len(sizes),
+                                      ctypes.byref(first)))
+        return first.value
+
+    def store_remove(self, first_id):
+        check(lib().msa_srv_store_remove(self.handle, int(first_id)))
+
+    def submit_resident(self, ids):
+        """An all-pairs job over resident sequences, in the order of `ids`."""
+        ids = np.ascontiguousarray(ids, dtype=np.int64)
+        job = ctypes.c_uint64(0)
+        check(lib().msa_srv_submit_resident(self.handle, _ptr(ids), len(ids), ctypes.byref(job)))
+        n = len(ids)
+        self.npairs[job.value] = n * (n - 1) // 2
+        return job.value
+
+    def fetch_device(self, job, d_out, capacity, stream=None):
+        """Queues a copy of a finished job's scores to device memory `d_out`
+        on `stream`; returns the stats dict."""
+        st = SrvJobStats()
+        rc = lib().msa_srv_fetch_device(self.handle, job, d_out, int(capacity), stream, ctypes.byref(st))
+        if rc not in (MSA_ERR_NOT_READY, MSA_ERR_INVALID_ARGUMENT):
+            self.npairs.pop(job, None)
+        check(rc)
+        return {f: getattr(st, f) for f, _ in SrvJobStats._fields_}
 
     def done(self, job):
         flag = ctypes.c_int(0)

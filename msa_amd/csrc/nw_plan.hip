@@ -192,7 +192,7 @@ int long_threshold_of(const msa_nw_options& opt, int32_t* threshold)
 
 int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const int32_t* size, int32_t nseq,
               const int32_t* pairs, int64_t npairs, const int32_t* table625, int32_t penalty,
-              const msa_nw_options& opt, const Tuning* tuning_override)
+              const msa_nw_options& opt, const Tuning* tuning_override, bool copy_blocks)
 {
     if (nseq < 0 || npairs < 0) return fail(MSA_ERR_INVALID_ARGUMENT, "negative sizes");
     if (nseq > 0 && (!offset || !size || !blocks)) return fail(MSA_ERR_INVALID_ARGUMENT, "null sequence database");
@@ -380,7 +380,7 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
 
     // Sequence store, with the views rebased as the reference's to_device does
     // (src/pairwise/database.cu:56-72).
-    hp.blocks.assign(blocks, blocks + total_blocks);
+    if (copy_blocks) hp.blocks.assign(blocks, blocks + total_blocks);
     hp.seqinfo.resize(nseq);
     for (int32_t s = 0; s < nseq; ++s)
         hp.seqinfo[s] = make_int4((int)seqs[s].offset, seqs[s].blocks, seqs[s].first_end, seqs[s].last_real);
@@ -707,10 +707,10 @@ Scratch carve_scratch(const ScratchNeed& need, int32_t* base)
 }
 
 int launch_plan(const HostPlan& hp, const DataLayout& lay, const void* d_data, const Scratch& scr,
-                int32_t* d_out, hipStream_t stream)
+                int32_t* d_out, hipStream_t stream, const uint16_t* d_store)
 {
     const char* base = static_cast<const char*>(d_data);
-    const uint16_t* d_blocks = reinterpret_cast<const uint16_t*>(base + lay.blocks);
+    const uint16_t* d_blocks = d_store ? d_store : reinterpret_cast<const uint16_t*>(base + lay.blocks);
     const SeqInfo* d_info = reinterpret_cast<const SeqInfo*>(base + lay.seqinfo);
     for (int c = 0; c < kTileConfigCount; ++c) {
         const int64_t first = hp.batch_first[c], n = hp.batch_first[c + 1] - first;

@@ -169,9 +169,12 @@ struct HostPlan {
 // Builds a HostPlan for the current device.  `pairs` may be null (all pairs
 // in reference order).  Input errors are reported before any device call.
 // `tuning`, when not null, replaces the table the options name.
+// `copy_blocks` false leaves hp.blocks empty: the sequences are already on the
+// device at the same offsets (a resident store) and the data block carries
+// only the work lists.
 int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const int32_t* size, int32_t nseq,
               const int32_t* pairs, int64_t npairs, const int32_t* table625, int32_t penalty,
-              const msa_nw_options& opt, const Tuning* tuning = nullptr);
+              const msa_nw_options& opt, const Tuning* tuning = nullptr, bool copy_blocks = true);
 
 // msa_nw_plan_create with an explicit tuning table (null: from the options).
 // One execution of a plan on `stream` (graph or eager), with the plan's lock
@@ -214,8 +217,9 @@ struct Scratch {
 };
 Scratch carve_scratch(const ScratchNeed& need, int32_t* base);
 
-// Launches every kernel of one execution of `hp` (data at `d_data`).
+// Launches every kernel of one execution of `hp` (data at `d_data`; the
+// sequence blocks at `d_store` instead when it is not null).
 int launch_plan(const HostPlan& hp, const DataLayout& lay, const void* d_data, const Scratch& scratch,
-                int32_t* d_out, hipStream_t stream);
+                int32_t* d_out, hipStream_t stream, const uint16_t* d_store = nullptr);
 
 }  // namespace msa

@@ -1,5 +1,6 @@
 """The serving loop (msa_srv_*): jobs admitted as they arrive, overlapping on
 the device, each job's scores identical alone and in the mix."""
+import ctypes
 import threading
 import time
 
@@ -170,5 +171,118 @@ def test_serving_workload_poisson(rate):
         if rate >= 500:
             assert overlaps >= 1
         assert srv.stats()["completed"] == len(jobs)
+    finally:
+        srv.close()
+
+
+def _concat(dbs):
+    """One store holding the sequences of several stores, in order."""
+    blocks, offsets, sizes, base = [], [], [], 0
+    for b, o, s in dbs:
+        blocks.append(np.asarray(b, dtype=np.uint16))
+        offsets.append(np.asarray(o, dtype=np.int64) + base)
+        sizes.append(np.asarray(s, dtype=np.int32))
+        base += len(b)
+    return np.concatenate(blocks), np.concatenate(offsets), np.concatenate(sizes)
+
+
+def _select(db, idx):
+    b, o, s = db
+    return b, np.asarray(o)[idx], np.asarray(s)[idx]
+
+
+@pytest.mark.gpu
+def test_resident_store_jobs_upload_no_sequences():
+    """Sequences put once into the server's store and named by id: a resident
+    job uploads only its work lists (no sequence bytes), its scores equal the
+    same sequences scored alone, also with ids drawn across two store_add
+    calls in any order and mixed with uploading jobs; the store's sets are
+    held while jobs read them and removed ids are refused."""
+    rng = np.random.default_rng(31)
+    T, p = H.tables()["blosum62"]
+    a = H.random_db(rng, 40, 50, 600, dirty=True)
+    b = H.random_db(rng, 30, 100, 900)
+    both = _concat([a, b])
+    srv = N.Server(T, p, slots=4, max_length=1024, store_bytes=8 << 20)
+    try:
+        first_a = srv.store_add(*a)
+        first_b = srv.store_add(*b)
+        assert first_b == first_a + 40
+        st = srv.stats()
+        assert st["store_sequences"] == 70 and st["store_bytes_used"] >= 2 * (len(a[0]) + len(b[0]))
+        picks = [np.arange(40), np.arange(40, 70), rng.permutation(70)[:25], rng.permutation(70)[:50],
+                 np.array([69, 0, 41, 3, 40, 2])]
+        jobs = []
+        for idx in picks:
+            jobs.append(("resident", idx, srv.submit_resident(first_a + idx)))
+            jobs.append(("upload", idx, srv.submit(*_select(both, idx))))
+        got = {}
+        for kind, idx, jid in jobs:
+            srv.wait(jid, 60000)
+            got[(kind, tuple(idx))] = srv.fetch(jid)
+        for kind, idx, _ in jobs:
+            scores, stats = got[(kind, tuple(idx))]
+            ref = N.align(*_select(both, idx), T, p)
+            assert np.array_equal(scores, ref), (kind, idx[:5])
+            if kind == "resident":
+                assert stats["sequence_bytes"] == 0
+                up = got[("upload", tuple(idx))][1]
+                assert up["sequence_bytes"] > 0
+                assert stats["upload_bytes"] <= up["upload_bytes"] - up["sequence_bytes"] + 256
+        # A set that a queued or running job reads cannot go; once the job
+        # has ended it can.
+        jid = srv.submit_resident(first_b + np.arange(30))
+        try:
+            srv.store_remove(first_b)
+            assert srv.done(jid)           # refused only while the job runs
+        except N.MsaError as e:
+            assert e.status == N.MSA_ERR_BUSY
+            srv.wait(jid, 60000)
+            srv.store_remove(first_b)
+        assert np.array_equal(srv.fetch(jid)[0], N.align(*b, T, p))
+        srv.store_remove(first_a)
+        assert srv.stats()["store_sequences"] == 0 and srv.stats()["store_bytes_used"] == 0
+        with pytest.raises(N.MsaError) as e:
+            srv.submit_resident(np.array([first_a, first_a + 1]))
+        assert e.value.status == N.MSA_ERR_NOT_FOUND
+        with pytest.raises(N.MsaError) as e:
+            srv.store_remove(first_a)
+        assert e.value.status == N.MSA_ERR_NOT_FOUND
+    finally:
+        srv.close()
+
+
+@pytest.mark.gpu
+def test_scores_stay_in_the_arena_until_fetched():
+    """A finished job's scores hold their arena block until fetched, to the
+    host or to device memory on a stream; both give the alone scores."""
+    rng = np.random.default_rng(32)
+    T, p = H.tables()["pam250"]
+    jobs = make_jobs(rng, 4, lo_n=20, hi_n=30, lo_len=100, hi_len=300, dirty_every=100)
+    alone = [N.align(*db, T, p) for db in jobs]
+    srv = N.Server(T, p, slots=2, max_length=512)
+    try:
+        ids = [srv.submit(*db) for db in jobs]
+        for jid in ids:
+            srv.wait(jid, 60000)
+        held = srv.stats()["device_bytes_used"]
+        assert held >= sum(4 * len(r) for r in alone)
+        scores, st = srv.fetch(ids[0])
+        assert np.array_equal(scores, alone[0]) and st["fetch_us"] > 0
+        assert srv.stats()["device_bytes_used"] < held
+        L = N.lib()
+        stream = ctypes.c_void_p()
+        N.check(L.msa_stream_create(ctypes.byref(stream)))
+        for jid, ref in zip(ids[1:], alone[1:]):
+            buf = N.DeviceBuffer(4 * len(ref))
+            srv.fetch_device(jid, buf.ptr, len(ref), stream.value)
+            N.check(L.msa_stream_sync(stream.value))
+            assert np.array_equal(buf.download(np.int32, len(ref)), ref)
+            buf.close()
+        deadline = time.time() + 10
+        while srv.stats()["device_bytes_used"] and time.time() < deadline:
+            time.sleep(0.01)
+        assert srv.stats()["device_bytes_used"] == 0
+        N.check(L.msa_stream_destroy(stream.value))
     finally:
         srv.close()
