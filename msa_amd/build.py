@@ -9,6 +9,7 @@ Every HIP translation unit is compiled with packed FP32 instructions turned
 off (-Xclang -target-feature -Xclang -packed-fp32-ops); hipcc's host pass
 warns that it does not know that feature, which is harmless.
 """
+import json
 import os
 import subprocess
 import sys
@@ -35,28 +36,90 @@ def _run(cmd, cwd=None):
     subprocess.run(cmd, check=True, cwd=cwd)
 
 
+# The pairwise kernels' sources: their compiles also report each kernel's
+# registers, LDS and occupancy (the compiler's kernel-resource-usage remarks),
+# kept in kernel_resources.json next to the library for the tuner's record.
+RESOURCE_SOURCES = ("nw_batched.hip", "nw_batched_32x16.hip", "nw_batched_16x16.hip", "nw_plain.hip",
+                    "nw_long.hip")
+RESOURCE_FILE = os.path.join(PKG, "kernel_resources.json")
+_RESOURCE_KEYS = {"TotalSGPRs": "sgprs", "VGPRs": "vgprs", "AGPRs": "agprs", "ScratchSize [bytes/lane]": "scratch",
+                  "Occupancy [waves/SIMD]": "waves_per_simd", "VGPRs Spill": "vgpr_spill",
+                  "LDS Size [bytes/block]": "static_lds"}
+
+
+def parse_resource_remarks(text):
+    """{kernel name (demangled): {vgprs, agprs, sgprs, waves_per_simd, ...}}
+    from hipcc's -Rpass-analysis=kernel-resource-usage output."""
+    kernels, name = {}, None
+    for line in text.splitlines():
+        if "remark:" not in line:
+            continue
+        body = line.split("remark:", 1)[1].split(" [-Rpass", 1)[0].strip()
+        key, _, value = body.partition(": ")
+        if key == "Function Name":
+            name = value.strip()
+            kernels[name] = {}
+        elif name and key in _RESOURCE_KEYS:
+            try:
+                kernels[name][_RESOURCE_KEYS[key]] = int(value)
+            except ValueError:
+                pass
+    if kernels:
+        try:
+            names = list(kernels)
+            out = subprocess.run(["c++filt"], input="\n".join(names), capture_output=True, text=True, check=True)
+            kernels = dict(zip(out.stdout.splitlines(), kernels.values()))
+        except (OSError, subprocess.CalledProcessError):
+            pass
+    return kernels
+
+
 def build_hip(jobs=None):
     jobs = jobs or max(1, min(8, os.cpu_count() or 1))
     objdir = os.path.join(PKG, "_build")
     os.makedirs(objdir, exist_ok=True)
     objs = []
     procs = []
+    remarks = []
+
+    def reap(proc, src, log):
+        rc = proc.wait()
+        if log is not None:
+            log.close()
+            with open(log.name) as f:
+                text = f.read()
+            other = [ln for ln in text.splitlines() if "remark:" not in ln and ln.strip() and
+                     not ln.lstrip().startswith(("|", "^")) and not ln.strip()[0].isdigit()]
+            if other:
+                print("\n".join(other), file=sys.stderr, flush=True)
+            remarks.append(text)
+        if rc != 0:
+            raise RuntimeError("hipcc failed on %s" % src)
+
     for src in HIP_SOURCES:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
         objs.append(obj)
         cmd = [HIPCC] + HIP_FLAGS + ["-I", INCLUDE, "-I", CSRC, "-c", os.path.join(CSRC, src), "-o", obj]
+        log = None
+        if src in RESOURCE_SOURCES:
+            cmd.append("-Rpass-analysis=kernel-resource-usage")
+            log = open(obj + ".remarks", "w")
         print("+", " ".join(cmd), flush=True)
-        procs.append(subprocess.Popen(cmd))
+        procs.append((subprocess.Popen(cmd, stderr=log), src, log))
         if len(procs) >= jobs:
-            if procs.pop(0).wait() != 0:
-                raise RuntimeError("hipcc failed")
-    for p in procs:
-        if p.wait() != 0:
-            raise RuntimeError("hipcc failed")
+            reap(*procs.pop(0))
+    for entry in procs:
+        reap(*entry)
     out = os.path.join(PKG, "libmsa_hip.so")
     tmp = out + ".tmp"
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + ["-pthread"])
     os.replace(tmp, out)
+    resources = {}
+    for text in remarks:
+        resources.update(parse_resource_remarks(text))
+    with open(RESOURCE_FILE, "w") as f:
+        json.dump({"arch": ARCH, "source": "hipcc -Rpass-analysis=kernel-resource-usage", "kernels": resources}, f,
+                  indent=1)
     return out
 
 
