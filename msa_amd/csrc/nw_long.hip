@@ -162,6 +162,37 @@ __device__ __forceinline__ int col_slot(const Target& t, const dev::lds_u8* rema
     return (idx >= 0 && idx < t.len) ? (int)remap[dev::code_at(t.seq, idx)] : null_slot;
 }
 
+// A lane's column of the target, chunk after chunk (idx = chunk*64 + k -
+// lpad), as block index and bit shift advanced by additions (a chunk moves 64
+// columns: 21 blocks and one 5-bit slot).  The block is loaded at the start of
+// a chunk and decoded after its steps: a load used at once would make the
+// wave wait for it and, the wait counter being shared with stores, for every
+// line store the chunk has issued.
+struct SlotCursor {
+    int idx;
+    int blk;
+    int shift;      // 1 + 5 * (idx mod 3)
+
+    __device__ __forceinline__ SlotCursor(const Target& t, int chunk, int k)
+    {
+        idx = chunk * kLanes + k - t.lpad;
+        const int q = (idx >= 0 ? idx : idx - 2) / 3;   // floor, also left of the target
+        blk = q;
+        shift = 1 + 5 * (idx - 3 * q);
+    }
+    __device__ __forceinline__ bool inside(const Target& t) const { return idx >= 0 && idx < t.len; }
+    __device__ __forceinline__ void advance()
+    {
+        idx += kLanes;
+        blk += kLanes / 3;
+        shift += 5 * (kLanes % 3);
+        if (shift > 11) {
+            shift -= 15;
+            blk += 1;
+        }
+    }
+};
+
 __device__ __forceinline__ void raise_error(int32_t* error)
 {
     __hip_atomic_store(error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -325,7 +356,12 @@ __device__ int run_strip(const dev::lds_u32* prof, const dev::lds_u8* remap, int
         for (int q = 0; q < W; ++q) w[q] = wn[q];
     };
 
+    SlotCursor cur(t, 2, k);    // the column decoded after chunk c's steps is chunk c+2's
     for (int c = 0; c < t.chunks && ok; ++c) {
+        // Chunk c+2's block, loaded before this chunk's steps and decoded
+        // after them.
+        const bool in2 = cur.inside(t);
+        const int raw2 = in2 ? (int)t.seq[cur.blk] : 0;
         const int base = (c * kLanes + 2 - k) & (RING - 1);
         volatile lds_i32* lcur = lline + (c & 1) * kLanes;
         volatile lds_i32* lnext = lline + ((c + 1) & 1) * kLanes;
@@ -350,7 +386,8 @@ __device__ int run_strip(const dev::lds_u32* prof, const dev::lds_u8* remap, int
             lnext[k] = lnv;
         }
         step(std::true_type {}, base, kLanes - 1, lcur, lnext);
-        cc_next = col_slot(t, remap, null_slot, c + 2, k);
+        cc_next = in2 ? (int)remap[(raw2 >> cur.shift) & 0x1F] : null_slot;
+        cur.advance();
         if constexpr (LOAD) raw_next = line_word(t, x.line_in, c + 2, k);
     }
     return h[R - 1];
