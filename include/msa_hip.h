@@ -144,6 +144,8 @@ typedef struct {
     const char* tuning_file; /* tuning table written by msa_nw_tune; NULL = the file named by the
                                 environment variable MSA_NW_TUNING when set, else the built-in
                                 table (16 lanes x 32 rows for every query, long from 8192) */
+    int single_query_strips; /* 1: every batched strip holds one query; 0 (default): queries
+                                that fill at most half a strip are stacked, several to a strip */
 } msa_nw_options;
 
 typedef struct {
@@ -155,6 +157,7 @@ typedef struct {
     int32_t launches;        /* kernel launches per execution */
     int32_t bins;            /* batched tile shapes in use (query-length bins after merging) */
     int64_t device_bytes;    /* device memory the plan holds */
+    int64_t stacked_pairs;   /* batched pairs scored in stacked strips (several queries to a strip) */
 } msa_nw_plan_info;
 
 typedef struct msa_nw_plan_opaque* msa_nw_plan;

@@ -73,8 +73,9 @@ static_assert(kTileConfigCount == 3, "one kernel source per tile configuration")
 struct BatchArgs {
     const uint16_t* blocks;
     const SeqInfo* info;
-    const int4* items;          // {query, first target, target count, flags}
-    const int2* targets;        // {sequence, output index}
+    const int4* items;          // {query or stack record, first target, target count, flags}
+    const int2* targets;        // {sequence, output index}; stacked items: {sequence, index in the
+                                // stack's union}, and the stack records and output tables
     int32_t nitems;
     int32_t* queue;
     int32_t* out;
@@ -174,6 +175,59 @@ __device__ __forceinline__ void build_profile(dev::lds_u32* prof, const dev::lds
     }
 }
 
+// Builds the profile of a stacked strip of R rows per lane (R a run-time
+// value: one instance serves every first-strip height): rows [0, pad_top)
+// neutral, then the members in order, each after a neutral separator row but
+// the first; a member's rows take its own orientation.  Same layout as
+// build_profile<G, R>.
+template <int G>
+__device__ __forceinline__ void build_profile_stacked(dev::lds_u32* prof, const dev::lds_u8* tab,
+                                                      const BatchArgs& a, int stack, int pad_top, int R, int lane)
+{
+    const int m = a.targets[stack].x;
+    const int W = (R + 3) / 4;
+    const int code_words_r = G * 4 * ((W + 3) / 4);
+    for (int slot = lane; slot < G * W; slot += 64) {
+        const int k = slot / W;
+        const int w = slot - k * W;
+        uint32_t word[kCodes];
+#pragma unroll
+        for (int b = 0; b < kCodes; ++b) word[b] = 0;
+        for (int t = 0; t < 4; ++t) {
+            const int u = k * R + 4 * w + t - pad_top;
+            if (u < 0 || 4 * w + t >= R) continue;
+            int start = 0;
+            for (int i = 0; i < m; ++i) {
+                const int lw = a.targets[stack + 2 + 2 * i].y;
+                const int lq = lw >> 1;
+                if (u < start + lq) {
+                    const int code = dev::code_at(a.blocks + a.info[a.targets[stack + 1 + 2 * i].x].x, u - start);
+                    if (code < kNull) {
+                        const bool flip = (lw & 1) != 0;
+#pragma unroll
+                        for (int b = 0; b < kNull; ++b)
+                            word[b] |= (uint32_t)(flip ? tab[b * 25 + code] : tab[code * 25 + b]) << (8 * t);
+                    }
+                    break;
+                }
+                if (u == start + lq) break;     // separator row: neutral
+                start += lq + 1;
+            }
+        }
+        dev::lds_u32* dst = prof + prof_word<G>(k, w);
+#pragma unroll
+        for (int b = 0; b < kCodes; ++b) dst[b * code_words_r] = word[b];
+    }
+}
+
+// What a lane's rows start from and which of them it reports.  Its rows begin
+// at K = k_lo, from row kb on at k_hi (a stacked member's separator); after
+// the pass it returns row sel (-1: none).  An unstacked strip starts every
+// row at 0 (the shifted boundary) and reports its last row on the last lane.
+struct LaneRows {
+    int k_lo, k_hi, kb, sel;
+};
+
 // The bottom-row store of a strip: the wave's line regions as one raw buffer
 // (dword 3 of the descriptor: 32-bit data format, as for any raw buffer
 // access on gfx9), and this lane's byte offset into it -- out of range
@@ -265,7 +319,7 @@ struct ColumnCursor {
 // lane 15.
 template <int G, int R, bool LOAD, bool STORE>
 __device__ __forceinline__ int run_strip(const dev::lds_u32* prof, const TargetState& t, int32_t* __restrict__ line,
-                         volatile dev::lds_u8* cring, const LineStore& ls, int wchunks, int k)
+                         volatile dev::lds_u8* cring, const LineStore& ls, int wchunks, int k, const LaneRows& lr)
 {
     static_assert(G == 16 || G == 32, "lane groups of 16 or 32");
     static_assert(code_words<G, R> * 4 <= kMaxTileBytes, "profile block exceeds the wave's region");
@@ -274,9 +328,8 @@ __device__ __forceinline__ int run_strip(const dev::lds_u32* prof, const TargetS
     constexpr int RING = 2 * G;
     int h[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) h[r] = 0;
+    for (int r = 0; r < R; ++r) h[r] = r < lr.kb ? lr.k_lo : lr.k_hi;
     int dtop = 0;
-    int result = 0;
 
     // Code ring: virtual column i at byte i mod 2G and again at i mod 2G + 2G;
     // columns -G..-1 are neutral, chunk 0 goes in before the first read.
@@ -349,26 +402,38 @@ __device__ __forceinline__ int run_strip(const dev::lds_u32* prof, const TargetS
             for (int q = 0; q < W; ++q) w[q] = wn[q];
         });
         if constexpr (LOAD) ln = ln_next;
-        result = (c == t.chunks - 1) ? h[R - 1] : result;
         cc_next = cur.code(t);
         if constexpr (LOAD) ln_next = cur.line(t, line);
         cur.advance();
     }
+    // Every row keeps its value over the neutral columns past the target (the
+    // code ring and the line give the neutral code and 0 there: a row's value
+    // is then the max of its own and the row above's, which is no larger), so
+    // the rows hold their final values after the wave's last chunk.
+    // The row compares stay here: hoisted above the loop, their 64-bit lane
+    // masks would hold R pairs of scalar registers across the strip.
+    int sel = lr.sel;
+    asm volatile("" : "+v"(sel));
+    int result = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) result = r == sel ? h[r] : result;
     return result;
 }
 
 // First strips: R0 rows per lane, R0 in kRowStep..RMAX.
 template <int G, int RMAX, bool STORE, int R = kRowStep>
 __device__ __forceinline__ int run_first_strip(int r0, const dev::lds_u32* prof, const TargetState& t, int32_t* line,
-                               volatile dev::lds_u8* cring, const LineStore& ls, int wchunks, int k)
+                               volatile dev::lds_u8* cring, const LineStore& ls, int wchunks, int k,
+                               const LaneRows& lr)
 {
     if constexpr (R >= RMAX) {
-        return run_strip<G, RMAX, false, STORE>(prof, t, line, cring, ls, wchunks, k);
+        return run_strip<G, RMAX, false, STORE>(prof, t, line, cring, ls, wchunks, k, lr);
     } else {
-        if (r0 == R) return run_strip<G, R, false, STORE>(prof, t, line, cring, ls, wchunks, k);
-        return run_first_strip<G, RMAX, STORE, R + kRowStep>(r0, prof, t, line, cring, ls, wchunks, k);
+        if (r0 == R) return run_strip<G, R, false, STORE>(prof, t, line, cring, ls, wchunks, k, lr);
+        return run_first_strip<G, RMAX, STORE, R + kRowStep>(r0, prof, t, line, cring, ls, wchunks, k, lr);
     }
 }
+
 
 template <int G, int RMAX, int R = kRowStep>
 __device__ __forceinline__ void build_profile_r(int r, dev::lds_u32* prof, const dev::lds_u8* tab, const uint16_t* qseq, int row0,
@@ -389,6 +454,35 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0)
     __builtin_amdgcn_wave_barrier();
+}
+
+// Lane k's rows of a stacked strip (R0 rows per lane, pad_top neutral rows
+// on top): member i > 0 starts its K at its separator row, and a lane holds at
+// most one separator and one member's last row (every member has at least a
+// lane's rows).  `member`: the member whose last row the lane reports, or -1.
+struct StackedRows {
+    LaneRows rows;
+    int member;
+};
+__device__ __forceinline__ StackedRows stacked_rows(const BatchArgs& a, int stack, int pad_top, int R0, int k)
+{
+    const int m = a.targets[stack].x;
+    StackedRows sr {{0, 0, R0, -1}, -1};
+    const int lo = k * R0;
+    int start = pad_top;
+    for (int i = 0; i < m; ++i) {
+        const int K = a.targets[stack + 1 + 2 * i].y;
+        const int lq = a.targets[stack + 2 + 2 * i].y >> 1;
+        if (i > 0) {
+            const int sep = start - 1;
+            if (sep <= lo) sr.rows.k_lo = K;
+            else if (sep < lo + R0) sr.rows.kb = sep - lo, sr.rows.k_hi = K;
+        }
+        const int last = start + lq - 1;
+        if (last >= lo && last < lo + R0) sr.rows.sel = last - lo, sr.member = i;
+        start += lq + 1;
+    }
+    return sr;
 }
 
 // All strips of one item with tile shape (G, RMAX).
@@ -412,17 +506,18 @@ __device__ __forceinline__ void run_item(const BatchArgs& a, const int4 item, de
                                                     kMaxGroups * a.line_stride * 4, kRawBufferFlags);
         ls.lane_off = k == G - 1 ? (uint32_t)(g * a.line_stride + kLineGuard) * 4u : kOutOfRange;
     }
-    const SeqInfo qi = a.info[item.x];
-    const int Lq = qi.z;
+    const bool stacked = (item.w & kStackedFlag) != 0;
+    const SeqInfo qi = a.info[stacked ? 0 : item.x];
+    const int Lq = stacked ? a.targets[item.x].y : qi.z;  // rows: the query's, or the stack's with its separators
     const uint16_t* qseq = a.blocks + qi.x;
     const bool transpose = (item.w & 1) != 0;
-    // Strips: a first strip of R0 rows per lane, then full strips of RMAX.
+    // Strips: a first strip of R0 rows per lane, then full strips of RMAX
+    // (a stack is one strip).
     const int full = (Lq - 1) / STRIP_MAX;            // strips after the first
     const int rem = Lq - full * STRIP_MAX;            // rows of the first strip
     const int R0 = min(RMAX, ((rem + G * kRowStep - 1) / (G * kRowStep)) * kRowStep);
     const int pad_top = G * R0 - rem;
     const int nst = full + 1;
-
     const int rounds = (item.z + GROUPS - 1) / GROUPS;
     for (int rd = 0; rd < rounds; ++rd) {
         const int ti = rd * GROUPS + g;
@@ -446,21 +541,40 @@ __device__ __forceinline__ void run_item(const BatchArgs& a, const int4 item, de
             const int row0 = st == 0 ? 0 : G * R0 + (st - 1) * STRIP_MAX;
             if (nst > 1 || rd == 0) {
                 wave_lds_sync();
-                build_profile_r<G, RMAX>(R, prof, tab, qseq, row0, pad_top, transpose, lane);
+                if (stacked) build_profile_stacked<G>(prof, tab, a, item.x, pad_top, R, lane);
+                else build_profile_r<G, RMAX>(R, prof, tab, qseq, row0, pad_top, transpose, lane);
                 wave_lds_sync();
             }
             const bool store = st + 1 < nst;
+            // An unstacked query reports its last row: the last lane's row
+            // R - 1 of the last strip.  A stacked lane's rows come from the
+            // stack (worked out per pass, so that nothing of it stays live
+            // across the strip).
+            LaneRows lr {0, 0, R, (st + 1 == nst && k == G - 1) ? R - 1 : -1};
+            if (stacked) lr = stacked_rows(a, item.x, pad_top, R0, k).rows;
             if (st == 0) {
-                res = store ? run_first_strip<G, RMAX, true>(R0, prof, t, line, cring, ls, wchunks, k)
-                            : run_first_strip<G, RMAX, false>(R0, prof, t, line, cring, ls, wchunks, k);
+                res = store ? run_first_strip<G, RMAX, true>(R0, prof, t, line, cring, ls, wchunks, k, lr)
+                            : run_first_strip<G, RMAX, false>(R0, prof, t, line, cring, ls, wchunks, k, lr);
             } else {
-                res = store ? run_strip<G, RMAX, true, true>(prof, t, line, cring, ls, wchunks, k)
-                            : run_strip<G, RMAX, true, false>(prof, t, line, cring, ls, wchunks, k);
+                res = store ? run_strip<G, RMAX, true, true>(prof, t, line, cring, ls, wchunks, k, lr)
+                            : run_strip<G, RMAX, true, false>(prof, t, line, cring, ls, wchunks, k, lr);
             }
             // The next strip reads this strip's bottom row from `line`.
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         }
-        if (tg.y >= 0 && k == G - 1) a.out[tg.y] = res - (Lq + t.len) * a.penalty;
+        if (tg.y >= 0) {
+            if (!stacked) {
+                if (k == G - 1) a.out[tg.y] = res - (Lq + t.len) * a.penalty;
+            } else {
+                const int mine = stacked_rows(a, item.x, pad_top, R0, k).member;
+                if (mine >= 0) {
+                    const int2 m0 = a.targets[item.x + 1 + 2 * mine];     // {query, K}
+                    const int2 m1 = a.targets[item.x + 2 + 2 * mine];     // {output row, rows * 2 + transpose}
+                    const int o = reinterpret_cast<const int32_t*>(a.targets)[m1.x + tg.y];
+                    if (o >= 0) a.out[o] = res - m0.y - ((m1.y >> 1) + t.len) * a.penalty;
+                }
+            }
+        }
     }
 }
 

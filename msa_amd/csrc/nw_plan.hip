@@ -47,10 +47,13 @@ size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 // 5 instructions (one more when the strip stores its bottom row, one more
 // when it loads the one above), on G lanes; every strip walks the target's
 // ceil((lt + 2G - 2) / G) * G steps.  A short query fills a lane's rows
-// poorly; a short target pays each strip's G - 1 steps of skew.
-int64_t query_weight(int64_t lq, const TileConfig& tc)
+// poorly; a short target pays each strip's G - 1 steps of skew.  A query that
+// will be stacked (`stacks`: at least RMAX rows and at most half a strip)
+// pays its share of a full strip: its rows and separator out of G * RMAX.
+int64_t query_weight(int64_t lq, const TileConfig& tc, bool stacks)
 {
     const int64_t g = tc.lanes, rmax = tc.rows;
+    if (stacks && lq >= rmax && 2 * (lq + 1) <= g * rmax) return ((2 * rmax + 5) * (lq + 1) + rmax - 1) / rmax;
     const int64_t full = (lq - 1) / (g * rmax);
     const int64_t rem = lq - full * g * rmax;
     const int64_t step = (int64_t)kRowStep * g;
@@ -253,7 +256,8 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
     for (int32_t s = 0; s < nseq; ++s) {
         const int cfg = forced >= 0 ? forced : tuning_config_for(*tuning, seqs[s].first_end);
         const int64_t len = seqs[s].first_end;
-        cterm[s] = {query_weight(len, kTileConfig[cfg]), kTileConfig[cfg].lanes, (int32_t)target_steps(len, 16),
+        cterm[s] = {query_weight(len, kTileConfig[cfg], opt.single_query_strips == 0), kTileConfig[cfg].lanes,
+                    (int32_t)target_steps(len, 16),
                     (int32_t)target_steps(len, 32)};
     }
     auto pair_cost = [&](int32_t q, int32_t t) {
@@ -398,6 +402,12 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
     // Batched part: group by (query, transpose), sort each query's targets by
     // length, cut into items of up to 16 targets, queue the items costliest
     // first.  Each item carries the tile shape of its query's length bin.
+    // Short queries are stacked: several single-strip queries of one tile
+    // shape, adjacent in length, share one strip (each below a neutral
+    // separator row, its rows offset by its own constant K so that the rows
+    // above cannot reach it) against the union of their target lists, with
+    // one output index per (member, target) and -1 where a member does not
+    // pair with a target.
     if (nfast > 0) {
         // Targets per item: 16 (four rounds of four) when the plan has enough
         // pairs to give every resident wave four items, else 8 or 4, so that
@@ -409,54 +419,212 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
         int item_targets = 16;
         while (item_targets > 4 && nfast < (int64_t)item_targets * 4 * waves) item_targets /= 2;
         struct Item { int4 v; int64_t cost; };
+        struct Group { int32_t part, key, cfg; int64_t begin, end; };
         struct PartItems {
             std::vector<Item> items;
+            std::vector<Group> stackable;
+            int64_t kept = 0;           // pairs of the part's groups that are not stacked
             int32_t max_t = 1;
             bool need_lines = false;
             unsigned used = 0;
         };
         std::vector<PartItems> pitems(parts.size());
-        hp.targets.resize((size_t)nfast);
-        // Every part holds whole queries, so parts build their items alone;
-        // an item's start is its offset in the whole target list.
-        const bool ok = run_parallel((int)parts.size(), [&](int t) {
+        // The largest sxx entry bounds a member's scores: K grows by
+        // lq * max_sxx + 1 from one member to the next.
+        int32_t max_sxx = 0;
+        for (int i = 0; i < 625; ++i) max_sxx = std::max<int32_t>(max_sxx, hp.sxx.v[i]);
+        const bool stacking = opt.single_query_strips == 0;
+        auto group_cfg = [&](int32_t lq) { return forced >= 0 ? forced : tuning_config_for(*tuning, lq); };
+        // A query stacks when it fills a strip poorly on its own (at most half
+        // of it) and holds at least a lane's rows, so that a lane sees at most
+        // one member's last row and one separator.
+        auto stackable = [&](int32_t lq, int cfg) {
+            const TileConfig tc = kTileConfig[cfg];
+            return stacking && lq >= tc.rows && 2 * ((int64_t)lq + 1) <= (int64_t)tc.lanes * tc.rows;
+        };
+        auto item_cost = [&](int64_t rows_in, const TileConfig& tc, const int2* tg, int32_t cnt) {
+            const int groups = 64 / tc.lanes;
+            const int64_t step = (int64_t)kRowStep * tc.lanes;
+            const int64_t rows = (rows_in + step - 1) / step * step;
+            int64_t cost = 0;
+            // A round of `groups` targets costs as much as its longest (the first).
+            for (int32_t u = 0; u < cnt; u += groups) {
+                const int64_t chunks = (seqs[tg[u].x].first_end + 2 * tc.lanes - 2) / tc.lanes;
+                cost += rows * chunks * tc.lanes;
+            }
+            return cost;
+        };
+        bool ok = run_parallel((int)parts.size(), [&](int t) {
             const std::vector<Fast>& fast = parts[t].fast;
             PartItems& out = pitems[t];
-            const int64_t base = fast_at[t];
-            for (size_t k = 0; k < fast.size(); ++k) {
-                hp.targets[base + k] = make_int2(fast[k].target, fast[k].index);
-                out.max_t = std::max(out.max_t, seqs[fast[k].target].first_end);
+            size_t k = 0;
+            while (k < fast.size()) {
+                size_t e = k;
+                while (e < fast.size() && fast[e].query == fast[k].query) ++e;
+                const int32_t lq = seqs[fast[k].query >> 1].first_end;
+                const int cfg = group_cfg(lq);
+                if (stackable(lq, cfg)) out.stackable.push_back({t, fast[k].query, cfg, (int64_t)k, (int64_t)e});
+                else out.kept += (int64_t)(e - k);
+                k = e;
             }
+        });
+        if (!ok) return fail(MSA_ERR_OUT_OF_MEMORY, "host allocation failed while planning");
+        // Stacks: the stackable groups of each tile shape in rank order (length
+        // descending), packed greedily while the strip holds them.
+        std::vector<Group> cand;
+        for (const PartItems& pi : pitems) cand.insert(cand.end(), pi.stackable.begin(), pi.stackable.end());
+        std::stable_sort(cand.begin(), cand.end(), [&](const Group& x, const Group& y) {
+            if (x.cfg != y.cfg) return x.cfg < y.cfg;
+            const int32_t rx = rank[x.key >> 1], ry = rank[y.key >> 1];
+            return rx != ry ? rx < ry : x.key < y.key;
+        });
+        struct Stack { size_t first; int32_t m, rows, cfg; int64_t uni, rec, at, souts; };
+        std::vector<Stack> stacks;
+        for (size_t g = 0; g < cand.size(); ++g) {
+            const TileConfig tc = kTileConfig[cand[g].cfg];
+            const int32_t lq = seqs[cand[g].key >> 1].first_end;
+            if (!stacks.empty()) {
+                Stack& st = stacks.back();
+                if (st.cfg == cand[g].cfg && st.m < kMaxStackMembers &&
+                    (int64_t)st.rows + 1 + lq <= (int64_t)tc.lanes * tc.rows) {
+                    st.rows += 1 + lq;
+                    ++st.m;
+                    continue;
+                }
+            }
+            stacks.push_back({g, 1, lq, cand[g].cfg, 0, 0, 0, 0});
+        }
+        // Union of each stack's target lists, in rank order; the j-th
+        // occurrence of a target in a member's list (an explicit list can
+        // name a pair twice) is its own union entry.
+        auto merge = [&](const Stack& st, int2* tg_out, int32_t* so_out) -> int64_t {
+            int64_t pos[kMaxStackMembers], occ[kMaxStackMembers];
+            for (int i = 0; i < st.m; ++i) pos[i] = cand[st.first + i].begin, occ[i] = 0;
+            int64_t u = 0;
+            while (true) {
+                int best = -1;
+                int32_t brank = 0;
+                int64_t bocc = 0;
+                for (int i = 0; i < st.m; ++i) {
+                    const Group& G = cand[st.first + i];
+                    if (pos[i] >= G.end) continue;
+                    const int32_t r = rank[parts[G.part].fast[pos[i]].target];
+                    if (best < 0 || r < brank || (r == brank && occ[i] < bocc)) best = i, brank = r, bocc = occ[i];
+                }
+                if (best < 0) return u;
+                const int32_t tgt = parts[cand[st.first + best].part].fast[pos[best]].target;
+                if (tg_out) tg_out[u] = make_int2(tgt, (int)u);
+                for (int i = 0; i < st.m; ++i) {
+                    const Group& G = cand[st.first + i];
+                    const std::vector<Fast>& f = parts[G.part].fast;
+                    const bool hit = pos[i] < G.end && f[pos[i]].target == tgt && occ[i] == bocc;
+                    if (so_out) so_out[(int64_t)i * st.uni + u] = hit ? f[pos[i]].index : -1;
+                    if (hit) {
+                        ++pos[i];
+                        occ[i] = pos[i] < G.end && f[pos[i]].target == tgt ? occ[i] + 1 : 0;
+                    }
+                }
+                ++u;
+            }
+        };
+        ok = run_parallel(std::min<int>((int)parts.size(), std::max<int>(1, (int)stacks.size())), [&](int t) {
+            const int T = std::min<int>((int)parts.size(), std::max<int>(1, (int)stacks.size()));
+            for (size_t s = (size_t)t; s < stacks.size(); s += (size_t)T) stacks[s].uni = merge(stacks[s], nullptr, nullptr);
+        });
+        if (!ok) return fail(MSA_ERR_OUT_OF_MEMORY, "host allocation failed while planning");
+        // Target list: the parts' unstacked groups in part order, then per
+        // stack its record, its union of targets and its output table (one
+        // row per member, in int32 units of the list).
+        std::vector<int64_t> kept_at(parts.size() + 1, 0);
+        for (size_t t = 0; t < parts.size(); ++t) kept_at[t + 1] = kept_at[t] + pitems[t].kept;
+        int64_t ntargets = kept_at.back();
+        for (Stack& st : stacks) {
+            st.rec = ntargets;
+            st.at = st.rec + 1 + 2 * (int64_t)st.m;
+            st.souts = 2 * (st.at + st.uni);
+            ntargets = st.at + st.uni + ((int64_t)st.m * st.uni + 1) / 2;
+        }
+        if (2 * ntargets >= (int64_t(1) << 31))
+            return fail(MSA_ERR_TOO_LARGE, "stacked target lists exceed 2^30 entries");
+        hp.targets.resize((size_t)ntargets);
+        // Every part holds whole queries, so parts build their items alone;
+        // an item's start is its offset in the whole target list.
+        ok = run_parallel((int)parts.size(), [&](int t) {
+            const std::vector<Fast>& fast = parts[t].fast;
+            PartItems& out = pitems[t];
+            int64_t at = kept_at[t];
             size_t k = 0;
             while (k < fast.size()) {
                 size_t e = k;
                 while (e < fast.size() && fast[e].query == fast[k].query) ++e;
                 const int32_t q = fast[k].query >> 1;
                 const int32_t lq = seqs[q].first_end;
-                const int cfg = forced >= 0 ? forced : tuning_config_for(*tuning, lq);
+                const int cfg = group_cfg(lq);
+                if (stackable(lq, cfg)) {
+                    k = e;
+                    continue;
+                }
                 const TileConfig tc = kTileConfig[cfg];
-                const int groups = 64 / tc.lanes;
                 out.used |= 1u << cfg;
                 out.need_lines = out.need_lines || lq > tc.lanes * tc.rows;
                 const int32_t flags = (fast[k].query & 1) | (cfg << 1);
-                // Rows computed: the query rounded up to kRowStep rows per lane.
-                const int64_t step = (int64_t)kRowStep * tc.lanes;
-                const int64_t rows = ((int64_t)lq + step - 1) / step * step;
-                for (size_t s = k; s < e; s += item_targets) {
-                    const int32_t cnt = (int32_t)std::min<size_t>(item_targets, e - s);
-                    int64_t cost = 0;
-                    // A round of `groups` targets costs as much as its longest (the first).
-                    for (size_t u = s; u < s + cnt; u += groups) {
-                        const int64_t chunks = (seqs[fast[u].target].first_end + 2 * tc.lanes - 2) / tc.lanes;
-                        cost += rows * chunks * tc.lanes;
-                    }
-                    out.items.push_back({make_int4(q, (int)(base + s), cnt, flags), cost});
+                const int64_t first = at;
+                for (size_t u = k; u < e; ++u) {
+                    hp.targets[at++] = make_int2(fast[u].target, fast[u].index);
+                    out.max_t = std::max(out.max_t, seqs[fast[u].target].first_end);
+                }
+                for (int64_t s = first; s < at; s += item_targets) {
+                    const int32_t cnt = (int32_t)std::min<int64_t>(item_targets, at - s);
+                    out.items.push_back({make_int4(q, (int)s, cnt, flags), item_cost(lq, tc, &hp.targets[s], cnt)});
                 }
                 k = e;
             }
-            std::vector<Fast>().swap(parts[t].fast);
         });
         if (!ok) return fail(MSA_ERR_OUT_OF_MEMORY, "host allocation failed while planning");
+        // Stacks: records, unions, output tables and items (stack s's items
+        // go to the part s % T).
+        const int TS = std::min<int>((int)parts.size(), std::max<int>(1, (int)stacks.size()));
+        std::vector<std::vector<Item>> sitems((size_t)TS);
+        std::vector<int64_t> spairs((size_t)TS, 0);
+        std::vector<int32_t> smax_t((size_t)TS, 1);
+        std::vector<unsigned> sused((size_t)TS, 0);
+        ok = run_parallel(TS, [&](int t) {
+            for (size_t s = (size_t)t; s < stacks.size(); s += (size_t)TS) {
+                const Stack& st = stacks[s];
+                const TileConfig tc = kTileConfig[st.cfg];
+                int32_t* outs = reinterpret_cast<int32_t*>(hp.targets.data()) + st.souts;
+                if ((int64_t)st.m * st.uni % 2) outs[(int64_t)st.m * st.uni] = -1;   // the last int2's padding
+                merge(st, &hp.targets[st.at], outs);
+                hp.targets[st.rec] = make_int2(st.m, st.rows);
+                int64_t K = 0;
+                for (int i = 0; i < st.m; ++i) {
+                    const Group& G = cand[st.first + i];
+                    const int32_t lq = seqs[G.key >> 1].first_end;
+                    hp.targets[st.rec + 1 + 2 * i] = make_int2(G.key >> 1, (int)K);
+                    hp.targets[st.rec + 2 + 2 * i] = make_int2((int)(st.souts + (int64_t)i * st.uni), lq * 2 + (G.key & 1));
+                    K += (int64_t)lq * max_sxx + 1;
+                    spairs[t] += G.end - G.begin;
+                }
+                sused[t] |= 1u << st.cfg;
+                const int32_t flags = (st.cfg << 1) | kStackedFlag;
+                for (int64_t u = 0; u < st.uni; u += item_targets) {
+                    const int32_t cnt = (int32_t)std::min<int64_t>(item_targets, st.uni - u);
+                    const int2* tg = &hp.targets[st.at + u];
+                    for (int32_t v = 0; v < cnt; ++v) smax_t[t] = std::max(smax_t[t], seqs[tg[v].x].first_end);
+                    sitems[t].push_back({make_int4((int)st.rec, (int)(st.at + u), cnt, flags),
+                                         item_cost(st.rows, tc, tg, cnt)});
+                }
+            }
+        });
+        if (!ok) return fail(MSA_ERR_OUT_OF_MEMORY, "host allocation failed while planning");
+        for (int t = 0; t < TS; ++t) {
+            PartItems& pi = pitems[(size_t)t];
+            pi.items.insert(pi.items.end(), sitems[t].begin(), sitems[t].end());
+            pi.max_t = std::max(pi.max_t, smax_t[t]);
+            pi.used |= sused[t];
+            hp.info.stacked_pairs += spairs[t];
+        }
+        for (Part& P : parts) std::vector<Fast>().swap(P.fast);
         std::vector<Item> items;
         int32_t max_t = 1;
         bool need_lines = false;

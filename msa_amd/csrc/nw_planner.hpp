@@ -32,6 +32,15 @@ constexpr int kTileConfigCount = 3;
 constexpr TileConfig kTileConfig[kTileConfigCount] = {{16, 32}, {16, 16}, {32, 16}};
 constexpr int kMaxTileBytes = 512;     // largest lanes*rows: one code's profile block
 constexpr int kRowStep = 2;            // rows-per-lane granularity of a query's first strip
+// Stacked items (bit 3 of an item's flags): item.x is the index, in the
+// target list, of the stack's record: {members m, rows of the stack (queries
+// plus separators)}, then per member {query, K} and {offset of its row of
+// output indices (in int32 units of the target list), rows * 2 + transpose};
+// the stack's targets are {sequence, index in the stack's target union}, and
+// its output table (per member, an output index per union target, -1 where
+// the member does not pair with it) sits in the target list too.
+constexpr int kStackedFlag = 8;
+constexpr int kMaxStackMembers = 16;
 // A batched lane group's strip-boundary line: kLineGuard ints in front of
 // column 0 and room for every column a strip's wavefront reaches (up to 2G
 // past the longest target), so that the bottom row is stored without bounds
@@ -152,7 +161,7 @@ struct HostPlan {
     int32_t plain_stride = 0;             // ints per boundary line
     // batched kernel
     std::vector<int4> items;              // grouped by tile configuration, costliest first in each
-    std::vector<int2> targets;
+    std::vector<int2> targets;            // also the stack records and output tables of stacked items
     int64_t batch_first[kTileConfigCount + 1] = {};   // items of configuration c: [first[c], first[c+1])
     int batch_grid[kTileConfigCount] = {};
     int32_t batch_stride = 0;

@@ -576,3 +576,71 @@ def test_reference_benchmark_databases(case):
     pairs = np.array([divmod_pair(int(c)) for c in idx], dtype=np.int32)
     exp = H.oracle_scores(db, w.table, w.penalty, pairs=pairs, workers=6)
     assert np.array_equal(auto[idx].astype(np.float32), exp)
+
+
+def plan_info(db, T, p, pairs=None, **kw):
+    plan = N.Plan(*db, T, p, pairs=pairs, opts=N.options(N.KERNEL_AUTO, **kw))
+    try:
+        return plan.info()
+    finally:
+        plan.close()
+
+
+@pytest.mark.parametrize("lanes,rows", TILES)
+def test_stacked_strips_every_tile_shape(lanes, rows):
+    """Short queries stacked several to a strip (each below a neutral
+    separator row, its rows offset by its own K), under every tile shape and
+    an asymmetric table (members of one stack in both orientations): every
+    score equals the oracle's and the unstacked plan's, and the plan reports
+    stacked pairs."""
+    rng = np.random.default_rng(91 + lanes + rows)
+    T, p = H.tables()["blosum62"]
+    Ta = T.copy()
+    Ta[0, 1], Ta[1, 0], Ta[6, 13], Ta[19, 4] = 7, -3, 2, -4
+    lengths = list(rng.integers(rows, lanes * rows // 2, 36)) + list(rng.integers(1, 2000, 8))
+    db = W.pack([rng.integers(0, 24, int(n)).astype(np.uint8) for n in lengths])
+    for table in (T, Ta):
+        got = assert_oracle(db, table, p, lanes=lanes, rows=rows)
+        flat = gpu_scores(db, table, p, lanes=lanes, rows=rows, stack=False)
+        assert np.array_equal(got, flat)
+        info = plan_info(db, table, p, lanes=lanes, rows=rows)
+        assert info.stacked_pairs > 0
+        assert plan_info(db, table, p, lanes=lanes, rows=rows, stack=False).stacked_pairs == 0
+
+
+def test_stacked_strips_explicit_list_with_repeats():
+    """An explicit pair list naming pairs twice, in both orders and out of
+    order, with queries that stack: every listed pair gets its own score
+    (a repeated (member, target) is its own union entry), equal to the
+    oracle and to the unstacked plan."""
+    rng = np.random.default_rng(57)
+    T, p = H.tables()["blosum62"]
+    db = H.random_db(rng, 30, 40, 300)
+    n = len(db[2])
+    pairs = [(int(a), int(b)) for a, b in rng.integers(0, n, (400, 2)) if a != b]
+    pairs += pairs[:60] + [(b, a) for a, b in pairs[60:120]]
+    pairs = np.array(pairs, dtype=np.int32)
+    got = assert_oracle(db, T, p, pairs=pairs)
+    assert np.array_equal(got, gpu_scores(db, T, p, pairs=pairs, stack=False))
+    assert plan_info(db, T, p, pairs=pairs).stacked_pairs > 0
+
+
+def test_stacked_strips_in_the_uneven_set_graph_replays():
+    """uneven-2000 with stacking: more than half of its pairs are scored in
+    stacked strips; graph replays equal the unstacked plan on every pair."""
+    w = W.uneven_batch(H.tables())
+    stacked = N.Plan(w.blocks, w.offsets, w.sizes, w.table, w.penalty, opts=N.options(N.KERNEL_AUTO, use_graph=True))
+    flat = N.Plan(w.blocks, w.offsets, w.sizes, w.table, w.penalty, opts=N.options(N.KERNEL_AUTO, stack=False))
+    out = N.DeviceBuffer(4 * w.npairs)
+    try:
+        assert stacked.info().stacked_pairs > w.npairs // 2
+        flat.execute(out.ptr)
+        exp = out.download(np.int32, w.npairs)
+        for _ in range(3):
+            out.upload(np.full(w.npairs, -7, dtype=np.int32))
+            stacked.execute(out.ptr)
+            assert np.array_equal(out.download(np.int32, w.npairs), exp)
+    finally:
+        out.close()
+        stacked.close()
+        flat.close()
