@@ -644,3 +644,20 @@ def test_stacked_strips_in_the_uneven_set_graph_replays():
         out.close()
         stacked.close()
         flat.close()
+
+
+def test_long_pair_beyond_the_line_limit_is_refused():
+    """The long kernel addresses each strip-boundary line through one buffer
+    descriptor whose byte size must fit 31 bits: a long pair whose target
+    needs a longer line is refused at planning (MSA_ERR_TOO_LARGE), before
+    anything is allocated, instead of scoring with wrapped offsets.  Two
+    sequences of 2^28 residues (one code throughout, blocks of zeros)."""
+    T, p = H.tables()["blosum62"]
+    nb = (2**28 + 2) // 3
+    blocks = np.zeros(2 * nb, dtype=np.uint16)
+    offsets = np.array([0, nb], dtype=np.int64)
+    sizes = np.array([nb, nb], dtype=np.int32)
+    with pytest.raises(N.MsaError) as e:
+        N.Plan(blocks, offsets, sizes, T, p, opts=N.options(N.KERNEL_AUTO))
+    assert e.value.status == N.MSA_ERR_TOO_LARGE
+    assert "boundary line" in str(e.value)
