@@ -67,6 +67,7 @@ constexpr int kRingBytes = 4 * kLanes;   // the wave's slot ring
 constexpr int kLineRingBytes = 2 * kLanes * 4;   // the wave's line ring: two chunks of ints
 constexpr int kSpinLimit = 1 << 24;     // ~1e9 cycles of waiting before giving up
 constexpr int kSc1 = 16;                // buffer-instruction cache policy: sc1 (agent scope)
+constexpr int kFirstStep = 8;           // rows-per-lane granularity of a pair's first strip
 
 struct LongArgs {
     const uint16_t* blocks;
@@ -393,6 +394,35 @@ __device__ int run_strip(const dev::lds_u32* prof, const dev::lds_u8* remap, int
     return h[R - 1];
 }
 
+// First strips: R0 rows per lane, R0 in kFirstStep..R (the first strip never
+// loads a line; it stores one when a strip follows it).
+template <int R, bool STORE, int R0 = kFirstStep>
+__device__ __forceinline__ int run_first_strip(int r0, const dev::lds_u32* prof, const dev::lds_u8* remap,
+                                               int null_slot, const Target& t, const Exchange& x, int32_t* error,
+                                               volatile dev::lds_u8* ring, volatile lds_i32* lline, int k, bool& ok)
+{
+    if constexpr (R0 >= R) {
+        return run_strip<R, false, STORE>(prof, remap, null_slot, t, x, error, ring, lline, k, ok);
+    } else {
+        if (r0 == R0) return run_strip<R0, false, STORE>(prof, remap, null_slot, t, x, error, ring, lline, k, ok);
+        return run_first_strip<R, STORE, R0 + kFirstStep>(r0, prof, remap, null_slot, t, x, error, ring, lline, k, ok);
+    }
+}
+
+template <int R, int R0 = kFirstStep>
+__device__ __forceinline__ void build_first_profile(int r0, dev::lds_u32* prof, const dev::lds_u8* tab,
+                                                    const dev::lds_u8* slot_code, int slots,
+                                                    const uint16_t* __restrict__ qseq, int pad_top, bool transpose,
+                                                    int lane)
+{
+    if constexpr (R0 >= R) {
+        build_profile<R>(prof, tab, slot_code, slots, qseq, 0, pad_top, transpose, lane);
+    } else {
+        if (r0 == R0) build_profile<R0>(prof, tab, slot_code, slots, qseq, 0, pad_top, transpose, lane);
+        else build_first_profile<R, R0 + kFirstStep>(r0, prof, tab, slot_code, slots, qseq, pad_top, transpose, lane);
+    }
+}
+
 // Dynamic LDS of one workgroup (one wave): the strip profile, the slot ring,
 // the line ring, the byte table, the code -> slot and slot -> code maps.
 template <int R>
@@ -430,7 +460,13 @@ __global__ __launch_bounds__(kLanes) void nw_long_kernel(LongArgs a)
         const SeqInfo ti = a.info[pr.y];
         const int Lq = qi.z;
         const int nst = (Lq + STRIP - 1) / STRIP;
-        const int pad_top = nst * STRIP - Lq;
+        // The first strip takes the rows left over by the full strips below
+        // it, kFirstStep rows per lane at a time, with at most 64 kFirstStep
+        // - 1 neutral padding rows on top.
+        const int rem = Lq - (nst - 1) * STRIP;
+        const int R0 = min(R, (rem + kLanes * kFirstStep - 1) / (kLanes * kFirstStep) * kFirstStep);
+        const int pad_top = kLanes * R0 - rem;
+        const int row0 = st == 0 ? 0 : kLanes * R0 + (st - 1) * STRIP;
         Target tg;
         tg.seq = a.blocks + ti.x;
         tg.len = ti.z;
@@ -442,7 +478,10 @@ __global__ __launch_bounds__(kLanes) void nw_long_kernel(LongArgs a)
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_s_waitcnt(0xC07F);
         __builtin_amdgcn_wave_barrier();
-        build_profile<R>(prof, tab, slot_code, a.slots, a.blocks + qi.x, st * STRIP, pad_top, pr.w != 0, lane);
+        if (st == 0)
+            build_first_profile<R>(R0, prof, tab, slot_code, a.slots, a.blocks + qi.x, pad_top, pr.w != 0, lane);
+        else
+            build_profile<R>(prof, tab, slot_code, a.slots, a.blocks + qi.x, row0, pad_top, pr.w != 0, lane);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_s_waitcnt(0xC07F);
         __builtin_amdgcn_wave_barrier();
@@ -472,9 +511,9 @@ __global__ __launch_bounds__(kLanes) void nw_long_kernel(LongArgs a)
         int res = 0;
         if (ok) {
             if (!load && !store)
-                res = run_strip<R, false, false>(prof, remap, null_slot, tg, x, a.error, ring, lline, lane, ok);
+                res = run_first_strip<R, false>(R0, prof, remap, null_slot, tg, x, a.error, ring, lline, lane, ok);
             else if (!load)
-                res = run_strip<R, false, true>(prof, remap, null_slot, tg, x, a.error, ring, lline, lane, ok);
+                res = run_first_strip<R, true>(R0, prof, remap, null_slot, tg, x, a.error, ring, lline, lane, ok);
             else if (store)
                 res = run_strip<R, true, true>(prof, remap, null_slot, tg, x, a.error, ring, lline, lane, ok);
             else

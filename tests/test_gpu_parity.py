@@ -123,6 +123,23 @@ def test_long_kernel_shapes(lo, hi, alphabet):
     assert_oracle(db, T, p, kernel="long")
 
 
+@pytest.mark.parametrize("alphabet", [4, 12, 24])
+def test_long_kernel_every_first_strip_height(alphabet):
+    """The long kernel's first strip takes the rows the full strips leave, in
+    steps of 8 rows per lane: query lengths in every step's range of residues
+    (one strip and two), for the 32-, 16- and 8-row strips that 4-, 12- and
+    24-code alphabets give; every pair against the oracle."""
+    rng = np.random.default_rng(5 + alphabet)
+    T, p = H.tables()["blosum62"]
+    rows = 32 if alphabet <= 7 else (16 if alphabet <= 15 else 8)
+    lengths = []
+    for r0 in range(8, rows + 1, 8):
+        rem = 64 * (r0 - 8) + 1 + int(rng.integers(0, 64 * 8))
+        lengths += [rem, rem + 64 * rows]
+    codes = [rng.integers(0, alphabet, n).astype(np.uint8) for n in lengths]
+    assert_oracle(W.pack(codes), T, p, kernel="long")
+
+
 def test_long_kernel_asymmetric_and_padding_routing():
     rng = np.random.default_rng(77)
     T, p = H.tables()["blosum62"]
