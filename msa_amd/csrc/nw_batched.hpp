@@ -62,7 +62,13 @@ constexpr int kNull = 24;
 constexpr int kWaves = 4;            // waves per workgroup (independent)
 constexpr int kMaxGroups = 4;        // lane groups per wave at 16 lanes
 constexpr int kProfWords = kCodes * kMaxTileBytes / 4;   // one wave's profile region
-constexpr int kTableBytes = 640;     // the 25x25 sxx table, padded to 16 bytes
+// The sxx table in LDS for the profile builders: rows of 28 bytes (the row
+// residue's code; columns: the column code), row 24 and columns 24..27 zero,
+// so that a neutral row and the neutral column code both read 0; a second
+// copy holds the transposed table.
+constexpr int kTabStride = 28;
+constexpr int kTabCopy = kCodes * kTabStride;                  // 700 bytes
+constexpr int kTableBytes = (2 * kTabCopy + 15) / 16 * 16;     // both copies, padded to 16 bytes
 constexpr int kRingBytes = 256;      // one wave's code rings: 4G bytes per lane group
 
 // Dynamic LDS of a workgroup: the waves' profile regions, the table, the rings.
@@ -142,44 +148,67 @@ __device__ __forceinline__ int prof_word(int k, int w)
     return (w >> 2) * G * 4 + k * 4 + (w & 3);
 }
 
-// Builds the profile of one strip: rows [row0, row0 + G*R) of the padded
-// query (rows below pad_top are neutral; bytes past a lane's R rows are 0).
-template <int G, int R>
-__device__ __forceinline__ void build_profile(dev::lds_u32* prof, const dev::lds_u8* tab,
-                              const uint16_t* __restrict__ qseq, int row0, int pad_top, bool transpose, int lane)
+// The profile words of one (lane, word) slot for every column code.  Byte t
+// of the word for code b is the table entry at byte row[t] + b of `tab` (row[t]
+// is the table row of the slot's row t: a row of either copy, or the zero row
+// 24 for a neutral row).  Four 4-byte table reads give a 4 x 4 block of bytes
+// (rows x codes 4j..4j+3), which eight byte permutes transpose into the words
+// of four codes.
+__device__ __forceinline__ void store_slot(dev::lds_u32* dst, int code_words, const dev::lds_u8* tab,
+                                           const int (&row)[4])
 {
-    constexpr int W = dev::prof_words<R>;
-    constexpr int CODE_WORDS = code_words<G, R>;
-    for (int slot = lane; slot < G * W; slot += 64) {
-        const int k = slot / W;
-        const int w = slot - k * W;
-        int codes[4];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int q = row0 + k * R + 4 * w + t - pad_top;
-            const int code = (q >= 0 && 4 * w + t < R) ? dev::code_at(qseq, q) : -1;
-            codes[t] = code < kNull ? code : -1;
-        }
-        dev::lds_u32* dst = prof + prof_word<G>(k, w);
-        for (int b = 0; b < kCodes; ++b) {
-            uint32_t word = 0;
+    for (int j = 0; j < (kCodes + 3) / 4; ++j) {
+        const uint32_t a0 = *(const dev::lds_u32*)(tab + row[0] + 4 * j);
+        const uint32_t a1 = *(const dev::lds_u32*)(tab + row[1] + 4 * j);
+        const uint32_t a2 = *(const dev::lds_u32*)(tab + row[2] + 4 * j);
+        const uint32_t a3 = *(const dev::lds_u32*)(tab + row[3] + 4 * j);
+        // v_perm_b32: selector bytes 0-3 pick bytes of the second operand,
+        // 4-7 bytes of the first.
+        const uint32_t lo01 = __builtin_amdgcn_perm(a1, a0, 0x05010400u);   // a0.0 a1.0 a0.1 a1.1
+        const uint32_t hi01 = __builtin_amdgcn_perm(a1, a0, 0x07030602u);   // a0.2 a1.2 a0.3 a1.3
+        const uint32_t lo23 = __builtin_amdgcn_perm(a3, a2, 0x05010400u);
+        const uint32_t hi23 = __builtin_amdgcn_perm(a3, a2, 0x07030602u);
+        const uint32_t o[4] = {__builtin_amdgcn_perm(lo23, lo01, 0x05040100u),
+                               __builtin_amdgcn_perm(lo23, lo01, 0x07060302u),
+                               __builtin_amdgcn_perm(hi23, hi01, 0x05040100u),
+                               __builtin_amdgcn_perm(hi23, hi01, 0x07060302u)};
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int code = codes[t];
-                uint32_t v = 0;
-                if (code >= 0 && b < kNull) v = transpose ? tab[b * 25 + code] : tab[code * 25 + b];
-                word |= v << (8 * t);
-            }
-            dst[b * CODE_WORDS] = word;
-        }
+        for (int i = 0; i < 4; ++i)
+            if (4 * j + i < kCodes) dst[(4 * j + i) * code_words] = o[i];
     }
 }
 
-// Builds the profile of a stacked strip of R rows per lane (R a run-time
-// value: one instance serves every first-strip height): rows [0, pad_top)
+// Builds the profile of one strip of R rows per lane (R a run-time value:
+// one instance serves every strip height): rows [row0, row0 + G*R) of the
+// padded query (rows below pad_top are neutral; bytes past a lane's R rows
+// are 0).  Layout as load_prof<G, R> reads it.
+template <int G>
+__device__ __forceinline__ void build_profile(dev::lds_u32* prof, const dev::lds_u8* tab,
+                                              const uint16_t* __restrict__ qseq, int row0, int pad_top,
+                                              bool transpose, int R, int lane)
+{
+    const int W = (R + 3) / 4;
+    const int code_words_r = G * 4 * ((W + 3) / 4);
+    const int copy = transpose ? kTabCopy : 0;
+    for (int slot = lane; slot < G * W; slot += 64) {
+        const int w = slot / G;
+        const int k = slot % G;
+        int row[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int q = row0 + k * R + 4 * w + t - pad_top;
+            const int code = (q >= 0 && 4 * w + t < R) ? dev::code_at(qseq, q) : kNull;
+            row[t] = copy + min(code, kNull) * kTabStride;
+        }
+        store_slot(prof + prof_word<G>(k, w), code_words_r, tab, row);
+    }
+}
+
+// Builds the profile of a stacked strip of R rows per lane: rows [0, pad_top)
 // neutral, then the members in order, each after a neutral separator row but
 // the first; a member's rows take its own orientation.  Same layout as
-// build_profile<G, R>.
+// build_profile<G>.
 template <int G>
 __device__ __forceinline__ void build_profile_stacked(dev::lds_u32* prof, const dev::lds_u8* tab,
                                                       const BatchArgs& a, int stack, int pad_top, int R, int lane)
@@ -188,12 +217,11 @@ __device__ __forceinline__ void build_profile_stacked(dev::lds_u32* prof, const 
     const int W = (R + 3) / 4;
     const int code_words_r = G * 4 * ((W + 3) / 4);
     for (int slot = lane; slot < G * W; slot += 64) {
-        const int k = slot / W;
-        const int w = slot - k * W;
-        uint32_t word[kCodes];
-#pragma unroll
-        for (int b = 0; b < kCodes; ++b) word[b] = 0;
+        const int w = slot / G;
+        const int k = slot % G;
+        int row[4];
         for (int t = 0; t < 4; ++t) {
+            row[t] = kNull * kTabStride;
             const int u = k * R + 4 * w + t - pad_top;
             if (u < 0 || 4 * w + t >= R) continue;
             int start = 0;
@@ -202,21 +230,14 @@ __device__ __forceinline__ void build_profile_stacked(dev::lds_u32* prof, const 
                 const int lq = lw >> 1;
                 if (u < start + lq) {
                     const int code = dev::code_at(a.blocks + a.info[a.targets[stack + 1 + 2 * i].x].x, u - start);
-                    if (code < kNull) {
-                        const bool flip = (lw & 1) != 0;
-#pragma unroll
-                        for (int b = 0; b < kNull; ++b)
-                            word[b] |= (uint32_t)(flip ? tab[b * 25 + code] : tab[code * 25 + b]) << (8 * t);
-                    }
+                    row[t] = ((lw & 1) ? kTabCopy : 0) + min(code, kNull) * kTabStride;
                     break;
                 }
                 if (u == start + lq) break;     // separator row: neutral
                 start += lq + 1;
             }
         }
-        dev::lds_u32* dst = prof + prof_word<G>(k, w);
-#pragma unroll
-        for (int b = 0; b < kCodes; ++b) dst[b * code_words_r] = word[b];
+        store_slot(prof + prof_word<G>(k, w), code_words_r, tab, row);
     }
 }
 
@@ -435,18 +456,6 @@ __device__ __forceinline__ int run_first_strip(int r0, const dev::lds_u32* prof,
 }
 
 
-template <int G, int RMAX, int R = kRowStep>
-__device__ __forceinline__ void build_profile_r(int r, dev::lds_u32* prof, const dev::lds_u8* tab, const uint16_t* qseq, int row0,
-                                int pad_top, bool transpose, int lane)
-{
-    if constexpr (R >= RMAX) {
-        build_profile<G, RMAX>(prof, tab, qseq, row0, pad_top, transpose, lane);
-    } else {
-        if (r == R) build_profile<G, R>(prof, tab, qseq, row0, pad_top, transpose, lane);
-        else build_profile_r<G, RMAX, R + kRowStep>(r, prof, tab, qseq, row0, pad_top, transpose, lane);
-    }
-}
-
 // LDS accesses of one wave to its own region are ordered; this keeps the
 // compiler from moving them across the profile rebuild and waits for them.
 __device__ __forceinline__ void wave_lds_sync()
@@ -542,7 +551,7 @@ __device__ __forceinline__ void run_item(const BatchArgs& a, const int4 item, de
             if (nst > 1 || rd == 0) {
                 wave_lds_sync();
                 if (stacked) build_profile_stacked<G>(prof, tab, a, item.x, pad_top, R, lane);
-                else build_profile_r<G, RMAX>(R, prof, tab, qseq, row0, pad_top, transpose, lane);
+                else build_profile<G>(prof, tab, qseq, row0, pad_top, transpose, R, lane);
                 wave_lds_sync();
             }
             const bool store = st + 1 < nst;
@@ -588,7 +597,12 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(3))
     dev::lds_u32* prof = lds_base + wave * kProfWords;
     dev::lds_u8* tab = (dev::lds_u8*)(lds_base + kWaves * kProfWords);
     dev::lds_u8* rings = tab + kTableBytes + wave * kRingBytes;
-    for (int i = threadIdx.x; i < 640; i += kWaves * 64) tab[i] = a.sxx.v[i];
+    for (int i = threadIdx.x; i < 2 * kTabCopy; i += kWaves * 64) {
+        const int flip = i >= kTabCopy;
+        const int j = i - flip * kTabCopy;
+        const int r = j / kTabStride, c = j - r * kTabStride;
+        tab[i] = (r < kNull && c < kNull) ? a.sxx.v[flip ? c * 25 + r : r * 25 + c] : 0;
+    }
     __syncthreads();
 
     int32_t* lines = a.lines + (int64_t)(blockIdx.x * kWaves + wave) * kMaxGroups * a.line_stride;
