@@ -146,6 +146,11 @@ typedef struct {
                                 table (16 lanes x 32 rows for every query, long from 8192) */
     int single_query_strips; /* 1: every batched strip holds one query; 0 (default): queries
                                 that fill at most half a strip are stacked, several to a strip */
+    int packed_cells;        /* queries that fill one 16 x 32 strip on their own, when every score
+                                of their pairs fits 15 bits, score two targets per lane group in
+                                packed 16-bit cells: 0 (default) in plans large enough to give every
+                                resident wave items of 16 targets (a small plan keeps its single
+                                launch); 1 in every plan; -1 never (every batched cell an int32) */
 } msa_nw_options;
 
 typedef struct {
@@ -158,6 +163,7 @@ typedef struct {
     int32_t bins;            /* batched tile shapes in use (query-length bins after merging) */
     int64_t device_bytes;    /* device memory the plan holds */
     int64_t stacked_pairs;   /* batched pairs scored in stacked strips (several queries to a strip) */
+    int64_t packed_pairs;    /* batched pairs scored in packed 16-bit cells (two targets per lane) */
 } msa_nw_plan_info;
 
 typedef struct msa_nw_plan_opaque* msa_nw_plan;

@@ -30,13 +30,14 @@ int batched_groups_per_block() { return batched::kWaves * batched::kMaxGroups; }
 
 int batched_occupancy(int* blocks_per_cu)
 {
-    // The smallest of the three kernels' occupancies: a plan sizes its grids
+    // The smallest of the four kernels' occupancies: a plan sizes its grids
     // and line buffers once for whichever configurations it uses.
-    int n[kTileConfigCount] = {};
+    int n[kBatchQueues] = {};
     MSA_TRY(batched::occupancy_16x32(&n[0]));
     MSA_TRY(batched::occupancy_16x16(&n[1]));
     MSA_TRY(batched::occupancy_32x16(&n[2]));
-    *blocks_per_cu = std::min({n[0], n[1], n[2]});
+    MSA_TRY(batched::occupancy_packed(&n[3]));
+    *blocks_per_cu = std::min({n[0], n[1], n[2], n[3]});
     return MSA_OK;
 }
 
@@ -44,7 +45,7 @@ int batched_launch(int config, int grid, const uint16_t* d_blocks, const SeqInfo
                    int32_t nitems, const int2* d_targets, int32_t* d_queue, int32_t* d_out, int32_t* d_lines,
                    int32_t line_stride, const Sxx& sxx, int32_t penalty, hipStream_t stream)
 {
-    if (config < 0 || config >= kTileConfigCount) return fail(MSA_ERR_INVALID_ARGUMENT, "no tile configuration %d", config);
+    if (config < 0 || config >= kBatchQueues) return fail(MSA_ERR_INVALID_ARGUMENT, "no batched queue %d", config);
     hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, stream, d_queue);
     MSA_HIP(hipGetLastError());
     batched::BatchArgs a;
@@ -62,7 +63,8 @@ int batched_launch(int config, int grid, const uint16_t* d_blocks, const SeqInfo
     switch (config) {
     case 0: return batched::launch_16x32(grid, a, stream);
     case 1: return batched::launch_16x16(grid, a, stream);
-    default: return batched::launch_32x16(grid, a, stream);
+    case 2: return batched::launch_32x16(grid, a, stream);
+    default: return batched::launch_packed(grid, a, stream);
     }
 }
 

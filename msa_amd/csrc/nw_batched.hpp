@@ -69,12 +69,20 @@ constexpr int kProfWords = kCodes * kMaxTileBytes / 4;   // one wave's profile r
 constexpr int kTabStride = 28;
 constexpr int kTabCopy = kCodes * kTabStride;                  // 700 bytes
 constexpr int kTableBytes = (2 * kTabCopy + 15) / 16 * 16;     // both copies, padded to 16 bytes
-constexpr int kRingBytes = 256;      // one wave's code rings: 4G bytes per lane group
+// One wave's code rings: per lane group 4G bytes (one ring of 2G columns,
+// twice), 256 bytes a wave; in the packed kernel 6G bytes (two rings of 3G
+// bytes, one per target), 384 bytes a wave of four 16-lane groups.
+constexpr int kRingBytes = 256;
+constexpr int kPackedRingBytes = 384;
 
 // Dynamic LDS of a workgroup: the waves' profile regions, the table, the rings.
-constexpr size_t lds_bytes() { return (size_t)kWaves * kProfWords * 4 + kTableBytes + kWaves * kRingBytes; }
+constexpr size_t lds_bytes(bool packed = false)
+{
+    return (size_t)kWaves * kProfWords * 4 + kTableBytes + kWaves * (packed ? kPackedRingBytes : kRingBytes);
+}
 
 static_assert(kTileConfigCount == 3, "one kernel source per tile configuration");
+static_assert(kPackedQueue == 3 && kBatchQueues == 4, "the packed kernel is the fourth batched queue");
 
 struct BatchArgs {
     const uint16_t* blocks;
@@ -587,6 +595,192 @@ __device__ __forceinline__ void run_item(const BatchArgs& a, const int4 item, de
     }
 }
 
+// ---- Packed cells: two targets per lane group ---------------------------
+//
+// For a query that fills one 16 x 32 strip on its own, every cell of the
+// shifted recurrence lies in [0, max(sxx) * rows]; the planner sends such a
+// query here only when that bound is at most 0x7BFF.  A lane then holds a
+// row of two targets in one register, target A in the low 16 bits and target
+// B in the high 16 bits, and one step of a row is three instructions for two
+// cells: v_perm_b32 takes the row's profile byte for A's column code and for
+// B's into the two halves, v_pk_add_u16 adds them to the diagonal cells, and
+// v_pk_maximum3_f16 takes the max3.  The f16 maximum of 16-bit patterns in
+// 0x0000..0x7BFF (positive finite f16 values, denormals included) is the
+// integer maximum of the patterns, so every cell is the exact integer of the
+// int32 recurrence.
+
+// Max of three packed cells (the f16 maximum of their bit patterns).
+__device__ __forceinline__ uint32_t pk_max3(uint32_t a, uint32_t b, uint32_t c)
+{
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    const h2 x = __builtin_bit_cast(h2, a), y = __builtin_bit_cast(h2, b), z = __builtin_bit_cast(h2, c);
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_maximum(__builtin_elementwise_maximum(x, y), z));
+}
+
+__device__ __forceinline__ uint32_t pk_add(uint32_t a, uint32_t b)
+{
+    typedef unsigned short u2 __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u2, a) + __builtin_bit_cast(u2, b));
+}
+
+// R packed rows of one lane at one column: wa / wb are the profile words of
+// A's and B's column codes (row r in byte r%4 of word r/4).
+template <int R>
+__device__ __forceinline__ void cells_packed(uint32_t (&h)[R], uint32_t up, uint32_t diag,
+                                             const uint32_t (&wa)[dev::prof_words<R>],
+                                             const uint32_t (&wb)[dev::prof_words<R>])
+{
+    uint32_t d = diag;
+    uint32_t u = up;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        // bytes: A's byte r%4, 0, B's byte r%4, 0
+        constexpr uint32_t kSel[4] = {0x0C040C00u, 0x0C050C01u, 0x0C060C02u, 0x0C070C03u};
+        const uint32_t s = __builtin_amdgcn_perm(wb[r >> 2], wa[r >> 2], kSel[r & 3]);
+        const uint32_t nh = pk_max3(pk_add(d, s), u, h[r]);
+        d = h[r];
+        h[r] = nh;
+        u = nh;
+    }
+}
+
+// One strip of R rows per lane (the whole query) against targets A and B.
+// Each target has its own code ring of 3G bytes: virtual column i at byte
+// i mod 2G, and columns with i mod 2G < G again at i mod 2G + 2G, so that a
+// lane reads the code for step u of a chunk at a per-chunk base plus u, as in
+// run_strip.  Returns both targets' last-row cells (last lane).
+template <int G, int R>
+__device__ __forceinline__ uint32_t run_strip_packed(const dev::lds_u32* prof, const TargetState& ta,
+                                                     const TargetState& tb, volatile dev::lds_u8* ra,
+                                                     volatile dev::lds_u8* rb, int wchunks, int k)
+{
+    static_assert(G == 16, "packed strips use 16-lane groups");
+    constexpr int W = dev::prof_words<R>;
+    constexpr int CODE_WORDS = code_words<G, R>;
+    constexpr int RING = 2 * G;
+    uint32_t h[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) h[r] = 0;
+    uint32_t dtop = 0;
+
+    // Columns -G..-1 are neutral; chunk 0 goes in before the first read.
+    ra[G + k] = kNull;
+    rb[G + k] = kNull;
+    {
+        const int ca = col_code<G>(ta, 0, k), cb = col_code<G>(tb, 0, k);
+        ra[k] = ca;
+        ra[RING + k] = ca;
+        rb[k] = cb;
+        rb[RING + k] = cb;
+    }
+    int na = col_code<G>(ta, 1, k), nb = col_code<G>(tb, 1, k);
+    ColumnCursor<G> ka(ta, 2, k), kb(tb, 2, k);
+
+    const int base0 = (RING - k) & (RING - 1);
+    int bna = ra[base0 + 1], bnb = rb[base0 + 1];
+    uint32_t wa[W], wb[W];
+    load_prof<G, R>(wa, prof + ra[base0] * CODE_WORDS, k);
+    load_prof<G, R>(wb, prof + rb[base0] * CODE_WORDS, k);
+
+    for (int c = 0; c < wchunks; ++c) {
+        const int base = (c * G + 2 - k) & (RING - 1);
+        static_for<0, G>([&](auto uc) {
+            constexpr int u = decltype(uc)::value;
+            // Chunk c+1 replaces chunk c-1, whose last reader was step G-4.
+            if constexpr (u == G - 3) {
+                const int slot = ((c + 1) * G + k) & (RING - 1);
+                ra[slot] = na;
+                rb[slot] = nb;
+                if (slot < G) {
+                    ra[RING + slot] = na;
+                    rb[RING + slot] = nb;
+                }
+            }
+            const int bnna = ra[base + u], bnnb = rb[base + u];
+            uint32_t wna[W], wnb[W];
+            load_prof<G, R>(wna, prof + bna * CODE_WORDS, k);
+            load_prof<G, R>(wnb, prof + bnb * CODE_WORDS, k);
+            const uint32_t hup = (uint32_t)dev::dpp_zero<dev::kRowShr1>((int)h[R - 1]);
+            cells_packed<R>(h, hup, dtop, wa, wb);
+            dtop = hup;
+            bna = bnna;
+            bnb = bnnb;
+#pragma unroll
+            for (int q = 0; q < W; ++q) {
+                wa[q] = wna[q];
+                wb[q] = wnb[q];
+            }
+        });
+        na = ka.code(ta);
+        nb = kb.code(tb);
+        ka.advance();
+        kb.advance();
+    }
+    return h[R - 1];
+}
+
+template <int R = 16>
+__device__ __forceinline__ uint32_t run_strip_packed_r(int r0, const dev::lds_u32* prof, const TargetState& ta,
+                                                       const TargetState& tb, volatile dev::lds_u8* ra,
+                                                       volatile dev::lds_u8* rb, int wchunks, int k)
+{
+    if constexpr (R >= 32) {
+        return run_strip_packed<16, 32>(prof, ta, tb, ra, rb, wchunks, k);
+    } else {
+        if (r0 == R) return run_strip_packed<16, R>(prof, ta, tb, ra, rb, wchunks, k);
+        return run_strip_packed_r<R + kRowStep>(r0, prof, ta, tb, ra, rb, wchunks, k);
+    }
+}
+
+__device__ __forceinline__ TargetState target_state(const BatchArgs& a, int2 tg, bool valid)
+{
+    TargetState t {a.blocks, 0, 0, 0};
+    if (valid) {
+        const SeqInfo si = a.info[tg.x];
+        t.seq = a.blocks + si.x;
+        t.len = si.z;
+        t.chunks = (t.len + 2 * 16 - 2) / 16;
+        t.lpad = t.chunks * 16 - (t.len + 16 - 1);
+    }
+    return t;
+}
+
+// One packed item: a query of one 16 x 32 strip (225..512 rows) and up to 16
+// targets, eight at a time (two per lane group).
+__device__ __forceinline__ void run_item_packed(const BatchArgs& a, const int4 item, dev::lds_u32* prof,
+                                                const dev::lds_u8* tab, dev::lds_u8* rings, int lane)
+{
+    constexpr int G = 16, RMAX = 32, GROUPS = 4;
+    const int g = lane / G;
+    const int k = lane % G;
+    volatile dev::lds_u8* ra = rings + g * 6 * G;
+    volatile dev::lds_u8* rb = ra + 3 * G;
+    const SeqInfo qi = a.info[item.x];
+    const int Lq = qi.z;
+    const int R0 = min(RMAX, ((Lq + G * kRowStep - 1) / (G * kRowStep)) * kRowStep);
+    const int pad_top = G * R0 - Lq;
+    wave_lds_sync();
+    build_profile<G>(prof, tab, a.blocks + qi.x, 0, pad_top, (item.w & 1) != 0, R0, lane);
+    wave_lds_sync();
+    const int rounds = (item.z + 2 * GROUPS - 1) / (2 * GROUPS);
+    for (int rd = 0; rd < rounds; ++rd) {
+        const int ia = rd * 2 * GROUPS + 2 * g, ib = ia + 1;
+        const int2 tga = ia < item.z ? a.targets[item.y + ia] : make_int2(0, -1);
+        const int2 tgb = ib < item.z ? a.targets[item.y + ib] : make_int2(0, -1);
+        const TargetState ta = target_state(a, tga, ia < item.z), tb = target_state(a, tgb, ib < item.z);
+        int wchunks = max(ta.chunks, tb.chunks);
+        wchunks = max(wchunks, __shfl_xor(wchunks, 16));
+        wchunks = max(wchunks, __shfl_xor(wchunks, 32));
+        const uint32_t res = run_strip_packed_r(R0, prof, ta, tb, ra, rb, wchunks, k);
+        if (k == G - 1) {
+            if (tga.y >= 0) a.out[tga.y] = (int)(res & 0xFFFFu) - (Lq + ta.len) * a.penalty;
+            if (tgb.y >= 0) a.out[tgb.y] = (int)(res >> 16) - (Lq + tb.len) * a.penalty;
+        }
+        // The next round's rings are rewritten only after this round's reads.
+        wave_lds_sync();
+    }
+}
+
 template <int CFG>
 __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(3))) void nw_batched_kernel(BatchArgs a)
 {
@@ -596,7 +790,7 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(3))
     dev::lds_u32* lds_base = (dev::lds_u32*)lds;   // address-space cast: 32-bit LDS offsets
     dev::lds_u32* prof = lds_base + wave * kProfWords;
     dev::lds_u8* tab = (dev::lds_u8*)(lds_base + kWaves * kProfWords);
-    dev::lds_u8* rings = tab + kTableBytes + wave * kRingBytes;
+    dev::lds_u8* rings = tab + kTableBytes + wave * (CFG == kPackedQueue ? kPackedRingBytes : kRingBytes);
     for (int i = threadIdx.x; i < 2 * kTabCopy; i += kWaves * 64) {
         const int flip = i >= kTabCopy;
         const int j = i - flip * kTabCopy;
@@ -612,7 +806,8 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(3))
         if (lane == 0) it = atomicAdd(a.queue, 1);
         it = __shfl(it, 0);
         if (it >= a.nitems) break;
-        run_item<kTileConfig[CFG].lanes, kTileConfig[CFG].rows>(a, a.items[it], prof, tab, rings, lines, lane);
+        if constexpr (CFG == kPackedQueue) run_item_packed(a, a.items[it], prof, tab, rings, lane);
+        else run_item<kTileConfig[CFG].lanes, kTileConfig[CFG].rows>(a, a.items[it], prof, tab, rings, lines, lane);
     }
 }
 
@@ -621,7 +816,7 @@ template <int CFG>
 int kernel_occupancy(int* blocks_per_cu)
 {
     const void* fn = reinterpret_cast<const void*>(&nw_batched_kernel<CFG>);
-    const size_t lds = lds_bytes();
+    const size_t lds = lds_bytes(CFG == kPackedQueue);
     MSA_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int n = 0;
     MSA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kWaves * 64, lds));
@@ -632,7 +827,7 @@ int kernel_occupancy(int* blocks_per_cu)
 template <int CFG>
 int kernel_launch(int grid, const BatchArgs& a, hipStream_t stream)
 {
-    hipLaunchKernelGGL(nw_batched_kernel<CFG>, dim3(grid), dim3(kWaves * 64), lds_bytes(), stream, a);
+    hipLaunchKernelGGL(nw_batched_kernel<CFG>, dim3(grid), dim3(kWaves * 64), lds_bytes(CFG == kPackedQueue), stream, a);
     return hip_status(hipGetLastError(), "nw_batched_kernel launch");
 }
 
@@ -644,6 +839,8 @@ int occupancy_16x16(int* blocks_per_cu);
 int launch_16x16(int grid, const BatchArgs& a, hipStream_t stream);
 int occupancy_32x16(int* blocks_per_cu);
 int launch_32x16(int grid, const BatchArgs& a, hipStream_t stream);
+int occupancy_packed(int* blocks_per_cu);
+int launch_packed(int grid, const BatchArgs& a, hipStream_t stream);
 
 }  // namespace batched
 }  // namespace msa

@@ -427,6 +427,7 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
             int32_t max_t = 1;
             bool need_lines = false;
             unsigned used = 0;
+            int64_t packed = 0;         // pairs of the part's packed items
         };
         std::vector<PartItems> pitems(parts.size());
         // The largest sxx entry bounds a member's scores: K grows by
@@ -441,6 +442,19 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
         auto stackable = [&](int32_t lq, int cfg) {
             const TileConfig tc = kTileConfig[cfg];
             return stacking && lq >= tc.rows && 2 * ((int64_t)lq + 1) <= (int64_t)tc.lanes * tc.rows;
+        };
+        // A query takes the packed queue when it fills one 16 x 32 strip on
+        // its own (225..512 rows: first-strip heights of 16..32 rows per lane)
+        // and is not stacked, and every score of its pairs fits a packed
+        // cell: in the shifted domain a strip's cells lie in [0, max(sxx) *
+        // rows].
+        // Packed items run in a launch of their own: a plan too small to give
+        // every resident wave items of 16 targets keeps its single launch.
+        const bool packing = opt.packed_cells > 0 || (opt.packed_cells == 0 && item_targets == 16);
+        auto packable = [&](int32_t lq, int cfg) {
+            const TileConfig tc = kTileConfig[cfg];
+            return packing && cfg == 0 && !stackable(lq, cfg) && lq >= kPackedMinRows &&
+                   lq <= tc.lanes * tc.rows && (int64_t)max_sxx * lq <= kPackedMaxScore;
         };
         auto item_cost = [&](int64_t rows_in, const TileConfig& tc, const int2* tg, int32_t cnt) {
             const int groups = 64 / tc.lanes;
@@ -567,7 +581,9 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
                 const TileConfig tc = kTileConfig[cfg];
                 out.used |= 1u << cfg;
                 out.need_lines = out.need_lines || lq > tc.lanes * tc.rows;
-                const int32_t flags = (fast[k].query & 1) | (cfg << 1);
+                const bool pk = packable(lq, cfg);
+                if (pk) out.packed += (int64_t)(e - k);
+                const int32_t flags = (fast[k].query & 1) | ((pk ? kPackedQueue : cfg) << 1);
                 const int64_t first = at;
                 for (size_t u = k; u < e; ++u) {
                     hp.targets[at++] = make_int2(fast[u].target, fast[u].index);
@@ -634,9 +650,10 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
             max_t = std::max(max_t, pi.max_t);
             need_lines = need_lines || pi.need_lines;
             used |= pi.used;
+            hp.info.packed_pairs += pi.packed;
         }
-        // One queue per tile configuration (each has its own kernel), every
-        // queue costliest first.  A stable sort, so the order is unique: the
+        // One queue per tile configuration and the packed queue (each has its
+        // own kernel), every queue costliest first.  A stable sort, so the order is unique: the
         // planning threads sort contiguous ranges, then adjacent ranges are
         // merged (stably) level by level.
         auto before = [](const Item& x, const Item& y) {
@@ -666,7 +683,7 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
             hp.items[u] = items[u].v;
             ++hp.batch_first[((items[u].v.w >> 1) & 3) + 1];
         }
-        for (int c = 0; c < kTileConfigCount; ++c) {
+        for (int c = 0; c < kBatchQueues; ++c) {
             const int64_t n = hp.batch_first[c + 1];
             hp.batch_first[c + 1] += hp.batch_first[c];
             hp.batch_grid[c] = (int)std::min<int64_t>((int64_t)per_cu * dev.compute_units, (n + 3) / 4);
@@ -786,7 +803,7 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
         hp.long_occupants = nlines_used > 0 ? (nlines_used + hp.long_nlines - 1) / hp.long_nlines : 0;
     }
     int batched_launches = 0;
-    for (int c = 0; c < kTileConfigCount; ++c) batched_launches += hp.batch_grid[c] > 0 ? 2 : 0;
+    for (int c = 0; c < kBatchQueues; ++c) batched_launches += hp.batch_grid[c] > 0 ? 2 : 0;
     hp.info.launches = (int32_t)(batched_launches + (hp.long_pairs.empty() ? 0 : 2) +
                                  (hp.plain_index.empty() ? 0 : 1));
     return MSA_OK;
@@ -844,7 +861,7 @@ ScratchNeed scratch_need(const HostPlan& hp)
     if (!hp.items.empty()) {
         // The configurations' launches run one after another on the stream
         // and share the lines.
-        const int grid = *std::max_element(hp.batch_grid, hp.batch_grid + kTileConfigCount);
+        const int grid = *std::max_element(hp.batch_grid, hp.batch_grid + kBatchQueues);
         n.batch_lines = up64((size_t)grid * batched_groups_per_block() * hp.batch_stride);
     }
     if (!hp.long_pairs.empty()) {
@@ -880,7 +897,7 @@ int launch_plan(const HostPlan& hp, const DataLayout& lay, const void* d_data, c
     const char* base = static_cast<const char*>(d_data);
     const uint16_t* d_blocks = d_store ? d_store : reinterpret_cast<const uint16_t*>(base + lay.blocks);
     const SeqInfo* d_info = reinterpret_cast<const SeqInfo*>(base + lay.seqinfo);
-    for (int c = 0; c < kTileConfigCount; ++c) {
+    for (int c = 0; c < kBatchQueues; ++c) {
         const int64_t first = hp.batch_first[c], n = hp.batch_first[c + 1] - first;
         if (n == 0) continue;
         MSA_TRY(batched_launch(c, hp.batch_grid[c], d_blocks, d_info,

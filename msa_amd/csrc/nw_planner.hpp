@@ -30,6 +30,15 @@ struct TileConfig {
 };
 constexpr int kTileConfigCount = 3;
 constexpr TileConfig kTileConfig[kTileConfigCount] = {{16, 32}, {16, 16}, {32, 16}};
+// Batched queues: one per tile configuration, and one more for the queries of
+// the 16 x 32 shape that fill one strip on their own and are not stacked,
+// when every score of their pairs fits 15 bits: their kernel scores two
+// targets per lane group in packed 16-bit cells.  An item's queue travels in
+// bits 1..2 of its flags.
+constexpr int kPackedQueue = kTileConfigCount;
+constexpr int kBatchQueues = kTileConfigCount + 1;
+constexpr int32_t kPackedMaxScore = 0x7BFF;   // largest packed cell value
+constexpr int kPackedMinRows = 225;            // packed queries: first-strip heights of 16..32 rows per lane
 constexpr int kMaxTileBytes = 512;     // largest lanes*rows: one code's profile block
 constexpr int kRowStep = 2;            // rows-per-lane granularity of a query's first strip
 // Stacked items (bit 3 of an item's flags): item.x is the index, in the
@@ -162,8 +171,8 @@ struct HostPlan {
     // batched kernel
     std::vector<int4> items;              // grouped by tile configuration, costliest first in each
     std::vector<int2> targets;            // also the stack records and output tables of stacked items
-    int64_t batch_first[kTileConfigCount + 1] = {};   // items of configuration c: [first[c], first[c+1])
-    int batch_grid[kTileConfigCount] = {};
+    int64_t batch_first[kBatchQueues + 1] = {};   // items of queue c: [first[c], first[c+1])
+    int batch_grid[kBatchQueues] = {};
     int32_t batch_stride = 0;
     // long kernel
     std::vector<int4> long_pairs;

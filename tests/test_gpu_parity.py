@@ -678,3 +678,75 @@ def test_long_pair_beyond_the_line_limit_is_refused():
         N.Plan(blocks, offsets, sizes, T, p, opts=N.options(N.KERNEL_AUTO))
     assert e.value.status == N.MSA_ERR_TOO_LARGE
     assert "boundary line" in str(e.value)
+
+
+@pytest.mark.parametrize("table", ["blosum62", "pam250", "default", "asymmetric"])
+def test_packed_cells_match_int32_and_oracle(table):
+    """Queries that fill one 16 x 32 strip on their own (225..512 rows) are
+    scored two targets per lane group in packed 16-bit cells: every score
+    equals the oracle's and the all-int32 plan's, with odd target counts
+    (a lane group with one target) and targets of every length."""
+    rng = np.random.default_rng(71 + len(table))
+    if table == "asymmetric":
+        T, p = H.tables()["blosum62"]
+        T = T.copy()
+        T[0, 1], T[1, 0], T[6, 13], T[19, 4] = 7, -3, 2, -4
+    else:
+        T, p = H.tables()[table]
+    lengths = list(rng.integers(225, 513, 30)) + list(rng.integers(1, 1500, 13))
+    db = W.pack([rng.integers(0, 24, int(n)).astype(np.uint8) for n in lengths])
+    got = assert_oracle(db, T, p, packed=True)
+    assert np.array_equal(got, gpu_scores(db, T, p, packed=False))
+    assert plan_info(db, T, p, packed=True).packed_pairs > 0
+    assert plan_info(db, T, p, packed=False).packed_pairs == 0
+    # a plan this small keeps its single launch unless packing is asked for
+    assert plan_info(db, T, p).packed_pairs == 0
+
+
+def test_packed_cells_at_the_top_of_their_range():
+    """A packed cell holds at most 0x7BFF: with max(sxx) = 61 a 512-row query
+    is packed (61 x 512 = 31,232), and a pair of two identical all-W
+    sequences reaches that bound; with max(sxx) = 62 (62 x 512 = 31,744) the
+    same query stays on int32 cells.  Scores equal the oracle's either way."""
+    T, _ = H.tables()["blosum62"]
+    w = W.letter_code("W")
+    assert T[w, w] == 11 and T.max() == 11
+    rng = np.random.default_rng(5)
+    seqs = [np.full(512, w, np.uint8), np.full(512, w, np.uint8), np.full(300, w, np.uint8)]
+    seqs += [rng.integers(0, 24, int(n)).astype(np.uint8) for n in rng.integers(225, 513, 6)]
+    db = W.pack(seqs)
+    for p, packed in ((25, True), (26, False)):
+        got = assert_oracle(db, T, p, packed=True)
+        assert int(got[0]) == 512 * 11             # pair (1, 0): the identical all-W pair
+        info = plan_info(db, T, p, packed=True)
+        if packed:
+            assert info.packed_pairs > 0
+        else:
+            # 62 x 512 > 0x7BFF: the 512-row queries stay on int32 cells
+            # (shorter ones may still be packed).
+            assert np.array_equal(got, gpu_scores(db, T, p, packed=False))
+    # 62 x 300 fits: the 300-row query is packed under p = 26 too
+    assert plan_info(db, T, 26, packed=True).packed_pairs > 0
+
+
+def test_packed_cells_in_the_uneven_set():
+    """uneven-2000: the default plan scores a share of its pairs in packed
+    cells, and every one of the 1,999,000 scores equals the all-int32 plan's
+    (graph replays included)."""
+    w = W.uneven_batch(H.tables())
+    packed = N.Plan(w.blocks, w.offsets, w.sizes, w.table, w.penalty, opts=N.options(N.KERNEL_AUTO, use_graph=True))
+    flat = N.Plan(w.blocks, w.offsets, w.sizes, w.table, w.penalty, opts=N.options(N.KERNEL_AUTO, packed=False))
+    out = N.DeviceBuffer(4 * w.npairs)
+    try:
+        assert packed.info().packed_pairs > 100000
+        assert flat.info().packed_pairs == 0
+        flat.execute(out.ptr)
+        exp = out.download(np.int32, w.npairs)
+        for _ in range(2):
+            out.upload(np.full(w.npairs, -7, dtype=np.int32))
+            packed.execute(out.ptr)
+            assert np.array_equal(out.download(np.int32, w.npairs), exp)
+    finally:
+        out.close()
+        packed.close()
+        flat.close()
