@@ -59,6 +59,9 @@ int batched_launch(int config, int grid, const uint16_t* d_blocks, const SeqInfo
     a.lines = d_lines;
     a.line_stride = line_stride;
     a.penalty = penalty;
+    int max_sxx = 0;
+    for (int i = 0; i < 625; ++i) max_sxx = std::max<int>(max_sxx, sxx.v[i]);
+    a.margin = 16 * max_sxx;
     a.sxx = sxx;
     switch (config) {
     case 0: return batched::launch_16x32(grid, a, stream);

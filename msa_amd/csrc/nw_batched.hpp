@@ -96,6 +96,7 @@ struct BatchArgs {
     int32_t* lines;             // one line of line_stride ints per lane group
     int32_t line_stride;
     int32_t penalty;
+    int32_t margin;             // packed kernel: 16 x the largest sxx (how far below the line a base is set)
     Sxx sxx;
 };
 
@@ -644,15 +645,59 @@ __device__ __forceinline__ void cells_packed(uint32_t (&h)[R], uint32_t up, uint
     }
 }
 
-// One strip of R rows per lane (the whole query) against targets A and B.
-// Each target has its own code ring of 3G bytes: virtual column i at byte
-// i mod 2G, and columns with i mod 2G < G again at i mod 2G + 2G, so that a
-// lane reads the code for step u of a chunk at a per-chunk base plus u, as in
-// run_strip.  Returns both targets' last-row cells (last lane).
-template <int G, int R>
+// Where a packed strip below another reads its two lines, and where a strip
+// with one below it stores its bottom rows: each target's line holds int32
+// scores of the shifted domain, as the int32 kernel's lines do.
+struct PackedLines {
+    const int32_t* la;          // target A's line, column 0
+    const int32_t* lb;          // target B's line
+    __amdgpu_buffer_rsrc_t rsrc;    // the wave's line regions
+    uint32_t off_a, off_b;      // this lane's byte offsets of A's and B's lines (out of range but on the last lane)
+};
+
+// A lane's line value of `t` at its column of chunk `chunk` (-1 outside the
+// target: its relative value is 0).
+__device__ __forceinline__ int packed_line_abs(const TargetState& t, const int32_t* __restrict__ line, int chunk, int k)
+{
+    const int idx = chunk * 16 + k - t.lpad;
+    return (idx >= 0 && idx < t.len) ? line[idx] : -1;
+}
+
+__device__ __forceinline__ uint32_t packed_rel(int abs_a, int base_a, int abs_b, int base_b)
+{
+    const uint32_t ra = abs_a < 0 ? 0u : (uint32_t)(abs_a - base_a);
+    const uint32_t rb = abs_b < 0 ? 0u : (uint32_t)(abs_b - base_b);
+    return ra | (rb << 16);
+}
+
+__device__ __forceinline__ uint32_t pk_sub(uint32_t a, uint32_t b)
+{
+    typedef unsigned short u2 __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u2, a) - __builtin_bit_cast(u2, b));
+}
+
+// One strip of R rows per lane against targets A and B.  LOAD: the strip
+// above left each target's bottom row in its line; STORE: this strip leaves
+// its bottom rows there (absolute int32 values, one buffer store per target
+// and step from the group's last lane).  A packed cell is the cell's value
+// minus its target's base.  A strip with no strip above never needs a base:
+// its cells are at most max(sxx) x rows.  A strip below another starts at
+// base 0 (its first columns' cells are at most max(sxx) x columns) and every
+// fourth chunk raises each base to the line value at the first lane's next
+// column minus `margin` (16 max(sxx)): every cell the group holds then is at
+// least the base (a cell is at least the line value above it, and the line
+// grows by at most max(sxx) a column), and until the next raise at most
+// 592 max(sxx) above it (rows x max(sxx) over the line, and the line's growth
+// over 4 chunks), which the planner keeps within 0x7BFF.  Each target has its
+// own code ring of 3G bytes: virtual column i at byte i mod 2G, and columns
+// with i mod 2G < G again at i mod 2G + 2G, so that a lane reads the code for
+// step u of a chunk at a per-chunk base plus u, as in run_strip.  Returns both
+// targets' last-row cells (last lane) and their bases.
+template <int G, int R, bool LOAD, bool STORE>
 __device__ __forceinline__ uint32_t run_strip_packed(const dev::lds_u32* prof, const TargetState& ta,
                                                      const TargetState& tb, volatile dev::lds_u8* ra,
-                                                     volatile dev::lds_u8* rb, int wchunks, int k)
+                                                     volatile dev::lds_u8* rb, const PackedLines& pl, int wchunks,
+                                                     int k, int margin, int& base_a, int& base_b)
 {
     static_assert(G == 16, "packed strips use 16-lane groups");
     constexpr int W = dev::prof_words<R>;
@@ -662,6 +707,8 @@ __device__ __forceinline__ uint32_t run_strip_packed(const dev::lds_u32* prof, c
 #pragma unroll
     for (int r = 0; r < R; ++r) h[r] = 0;
     uint32_t dtop = 0;
+    base_a = 0;
+    base_b = 0;
 
     // Columns -G..-1 are neutral; chunk 0 goes in before the first read.
     ra[G + k] = kNull;
@@ -675,6 +722,13 @@ __device__ __forceinline__ uint32_t run_strip_packed(const dev::lds_u32* prof, c
     }
     int na = col_code<G>(ta, 1, k), nb = col_code<G>(tb, 1, k);
     ColumnCursor<G> ka(ta, 2, k), kb(tb, 2, k);
+    uint32_t ln = 0;                // this chunk's line values, packed relative (lane k: column chunk*G + k)
+    int la_next = -1, lb_next = -1; // the next chunk's, absolute
+    if constexpr (LOAD) {
+        ln = packed_rel(packed_line_abs(ta, pl.la, 0, k), 0, packed_line_abs(tb, pl.lb, 0, k), 0);
+        la_next = packed_line_abs(ta, pl.la, 1, k);
+        lb_next = packed_line_abs(tb, pl.lb, 1, k);
+    }
 
     const int base0 = (RING - k) & (RING - 1);
     int bna = ra[base0 + 1], bnb = rb[base0 + 1];
@@ -684,6 +738,10 @@ __device__ __forceinline__ uint32_t run_strip_packed(const dev::lds_u32* prof, c
 
     for (int c = 0; c < wchunks; ++c) {
         const int base = (c * G + 2 - k) & (RING - 1);
+        // Byte offsets of the bottom cells the last lane makes at step 0 of
+        // this chunk (column c*G - (G-1) - lpad of each target).
+        const uint32_t va = pl.off_a + (uint32_t)(c * G - (G - 1) - ta.lpad) * 4u;
+        const uint32_t vb = pl.off_b + (uint32_t)(c * G - (G - 1) - tb.lpad) * 4u;
         static_for<0, G>([&](auto uc) {
             constexpr int u = decltype(uc)::value;
             // Chunk c+1 replaces chunk c-1, whose last reader was step G-4.
@@ -700,9 +758,19 @@ __device__ __forceinline__ uint32_t run_strip_packed(const dev::lds_u32* prof, c
             uint32_t wna[W], wnb[W];
             load_prof<G, R>(wna, prof + bna * CODE_WORDS, k);
             load_prof<G, R>(wnb, prof + bnb * CODE_WORDS, k);
-            const uint32_t hup = (uint32_t)dev::dpp_zero<dev::kRowShr1>((int)h[R - 1]);
+            uint32_t hup;
+            if constexpr (LOAD)
+                hup = (uint32_t)dev::dpp<dev::kRowShr1>(row_bcast<u>((int)ln), (int)h[R - 1]);
+            else
+                hup = (uint32_t)dev::dpp_zero<dev::kRowShr1>((int)h[R - 1]);
             cells_packed<R>(h, hup, dtop, wa, wb);
             dtop = hup;
+            if constexpr (STORE) {
+                const int va_abs = (int)(h[R - 1] & 0xFFFFu) + base_a;
+                const int vb_abs = (int)(h[R - 1] >> 16) + base_b;
+                __builtin_amdgcn_raw_buffer_store_b32((uint32_t)va_abs, pl.rsrc, (int)va, 4 * u, 0);
+                __builtin_amdgcn_raw_buffer_store_b32((uint32_t)vb_abs, pl.rsrc, (int)vb, 4 * u, 0);
+            }
             bna = bnna;
             bnb = bnnb;
 #pragma unroll
@@ -711,6 +779,25 @@ __device__ __forceinline__ uint32_t run_strip_packed(const dev::lds_u32* prof, c
                 wb[q] = wnb[q];
             }
         });
+        if constexpr (LOAD) {
+            if ((c & 3) == 3) {
+                // Raise the bases from the first lane's line values of the
+                // next chunk (its column (c+1)*G - lpad), where the target
+                // still runs.
+                const int fa = row_bcast<0>(la_next), fb = row_bcast<0>(lb_next);
+                const int nba = fa < 0 ? base_a : max(base_a, fa - margin);
+                const int nbb = fb < 0 ? base_b : max(base_b, fb - margin);
+                const uint32_t delta = (uint32_t)(nba - base_a) | ((uint32_t)(nbb - base_b) << 16);
+#pragma unroll
+                for (int r = 0; r < R; ++r) h[r] = pk_sub(h[r], delta);
+                dtop = pk_sub(dtop, delta);
+                base_a = nba;
+                base_b = nbb;
+            }
+            ln = packed_rel(la_next, base_a, lb_next, base_b);
+            la_next = packed_line_abs(ta, pl.la, c + 2, k);
+            lb_next = packed_line_abs(tb, pl.lb, c + 2, k);
+        }
         na = ka.code(ta);
         nb = kb.code(tb);
         ka.advance();
@@ -719,16 +806,19 @@ __device__ __forceinline__ uint32_t run_strip_packed(const dev::lds_u32* prof, c
     return h[R - 1];
 }
 
-template <int R = 16>
-__device__ __forceinline__ uint32_t run_strip_packed_r(int r0, const dev::lds_u32* prof, const TargetState& ta,
-                                                       const TargetState& tb, volatile dev::lds_u8* ra,
-                                                       volatile dev::lds_u8* rb, int wchunks, int k)
+template <bool STORE, int R = kRowStep>
+__device__ __forceinline__ uint32_t run_first_strip_packed(int r0, const dev::lds_u32* prof, const TargetState& ta,
+                                                           const TargetState& tb, volatile dev::lds_u8* ra,
+                                                           volatile dev::lds_u8* rb, const PackedLines& pl,
+                                                           int wchunks, int k, int margin, int& base_a, int& base_b)
 {
     if constexpr (R >= 32) {
-        return run_strip_packed<16, 32>(prof, ta, tb, ra, rb, wchunks, k);
+        return run_strip_packed<16, 32, false, STORE>(prof, ta, tb, ra, rb, pl, wchunks, k, margin, base_a, base_b);
     } else {
-        if (r0 == R) return run_strip_packed<16, R>(prof, ta, tb, ra, rb, wchunks, k);
-        return run_strip_packed_r<R + kRowStep>(r0, prof, ta, tb, ra, rb, wchunks, k);
+        if (r0 == R)
+            return run_strip_packed<16, R, false, STORE>(prof, ta, tb, ra, rb, pl, wchunks, k, margin, base_a, base_b);
+        return run_first_strip_packed<STORE, R + kRowStep>(r0, prof, ta, tb, ra, rb, pl, wchunks, k, margin, base_a,
+                                                           base_b);
     }
 }
 
@@ -745,23 +835,39 @@ __device__ __forceinline__ TargetState target_state(const BatchArgs& a, int2 tg,
     return t;
 }
 
-// One packed item: a query of one 16 x 32 strip (225..512 rows) and up to 16
-// targets, eight at a time (two per lane group).
+// One packed item: a query of the 16 x 32 shape and up to 16 targets, eight
+// at a time (two per lane group), strip after strip; a query of one strip
+// builds its profile once, a longer one per strip and round.  The wave's line
+// regions hold two lines per lane group (A's, then B's).
 __device__ __forceinline__ void run_item_packed(const BatchArgs& a, const int4 item, dev::lds_u32* prof,
-                                                const dev::lds_u8* tab, dev::lds_u8* rings, int lane)
+                                                const dev::lds_u8* tab, dev::lds_u8* rings, int32_t* lines, int lane)
 {
-    constexpr int G = 16, RMAX = 32, GROUPS = 4;
+    constexpr int G = 16, RMAX = 32, GROUPS = 4, STRIP_MAX = G * RMAX;
     const int g = lane / G;
     const int k = lane % G;
     volatile dev::lds_u8* ra = rings + g * 6 * G;
     volatile dev::lds_u8* rb = ra + 3 * G;
     const SeqInfo qi = a.info[item.x];
     const int Lq = qi.z;
-    const int R0 = min(RMAX, ((Lq + G * kRowStep - 1) / (G * kRowStep)) * kRowStep);
-    const int pad_top = G * R0 - Lq;
-    wave_lds_sync();
-    build_profile<G>(prof, tab, a.blocks + qi.x, 0, pad_top, (item.w & 1) != 0, R0, lane);
-    wave_lds_sync();
+    const uint16_t* qseq = a.blocks + qi.x;
+    const bool transpose = (item.w & 1) != 0;
+    const int full = (Lq - 1) / STRIP_MAX;
+    const int rem = Lq - full * STRIP_MAX;
+    const int R0 = min(RMAX, ((rem + G * kRowStep - 1) / (G * kRowStep)) * kRowStep);
+    const int pad_top = G * R0 - rem;
+    const int nst = full + 1;
+    PackedLines pl;
+    {
+        pl.la = lines + (int64_t)(2 * g) * a.line_stride + kLineGuard;
+        pl.lb = pl.la + a.line_stride;
+        const uint64_t wb = (uint64_t)(uintptr_t)lines;
+        const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)wb);
+        const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(wb >> 32));
+        pl.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)(((uint64_t)hi << 32) | lo), 0,
+                                                    2 * kMaxGroups * a.line_stride * 4, kRawBufferFlags);
+        pl.off_a = k == G - 1 ? (uint32_t)(2 * g * a.line_stride + kLineGuard) * 4u : kOutOfRange;
+        pl.off_b = k == G - 1 ? (uint32_t)((2 * g + 1) * a.line_stride + kLineGuard) * 4u : kOutOfRange;
+    }
     const int rounds = (item.z + 2 * GROUPS - 1) / (2 * GROUPS);
     for (int rd = 0; rd < rounds; ++rd) {
         const int ia = rd * 2 * GROUPS + 2 * g, ib = ia + 1;
@@ -771,10 +877,32 @@ __device__ __forceinline__ void run_item_packed(const BatchArgs& a, const int4 i
         int wchunks = max(ta.chunks, tb.chunks);
         wchunks = max(wchunks, __shfl_xor(wchunks, 16));
         wchunks = max(wchunks, __shfl_xor(wchunks, 32));
-        const uint32_t res = run_strip_packed_r(R0, prof, ta, tb, ra, rb, wchunks, k);
+        uint32_t res = 0;
+        int base_a = 0, base_b = 0;
+        for (int st = 0; st < nst; ++st) {
+            if (nst > 1 || rd == 0) {
+                wave_lds_sync();
+                const int row0 = st == 0 ? 0 : G * R0 + (st - 1) * STRIP_MAX;
+                build_profile<G>(prof, tab, qseq, row0, pad_top, transpose, st == 0 ? R0 : RMAX, lane);
+                wave_lds_sync();
+            }
+            const bool store = st + 1 < nst;
+            if (st == 0)
+                res = store ? run_first_strip_packed<true>(R0, prof, ta, tb, ra, rb, pl, wchunks, k, a.margin, base_a,
+                                                           base_b)
+                            : run_first_strip_packed<false>(R0, prof, ta, tb, ra, rb, pl, wchunks, k, a.margin, base_a,
+                                                            base_b);
+            else
+                res = store ? run_strip_packed<G, RMAX, true, true>(prof, ta, tb, ra, rb, pl, wchunks, k, a.margin,
+                                                                    base_a, base_b)
+                            : run_strip_packed<G, RMAX, true, false>(prof, ta, tb, ra, rb, pl, wchunks, k, a.margin,
+                                                                     base_a, base_b);
+            // The next strip reads this strip's bottom rows from the lines.
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        }
         if (k == G - 1) {
-            if (tga.y >= 0) a.out[tga.y] = (int)(res & 0xFFFFu) - (Lq + ta.len) * a.penalty;
-            if (tgb.y >= 0) a.out[tgb.y] = (int)(res >> 16) - (Lq + tb.len) * a.penalty;
+            if (tga.y >= 0) a.out[tga.y] = (int)(res & 0xFFFFu) + base_a - (Lq + ta.len) * a.penalty;
+            if (tgb.y >= 0) a.out[tgb.y] = (int)(res >> 16) + base_b - (Lq + tb.len) * a.penalty;
         }
         // The next round's rings are rewritten only after this round's reads.
         wave_lds_sync();
@@ -799,14 +927,16 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(3))
     }
     __syncthreads();
 
-    int32_t* lines = a.lines + (int64_t)(blockIdx.x * kWaves + wave) * kMaxGroups * a.line_stride;
+    // A wave's line regions: one line per lane group, two in the packed kernel.
+    int32_t* lines = a.lines + (int64_t)(blockIdx.x * kWaves + wave) * (CFG == kPackedQueue ? 2 : 1) * kMaxGroups *
+                               a.line_stride;
 
     while (true) {
         int it = 0;
         if (lane == 0) it = atomicAdd(a.queue, 1);
         it = __shfl(it, 0);
         if (it >= a.nitems) break;
-        if constexpr (CFG == kPackedQueue) run_item_packed(a, a.items[it], prof, tab, rings, lane);
+        if constexpr (CFG == kPackedQueue) run_item_packed(a, a.items[it], prof, tab, rings, lines, lane);
         else run_item<kTileConfig[CFG].lanes, kTileConfig[CFG].rows>(a, a.items[it], prof, tab, rings, lines, lane);
     }
 }

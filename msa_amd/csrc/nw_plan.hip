@@ -428,6 +428,7 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
             bool need_lines = false;
             unsigned used = 0;
             int64_t packed = 0;         // pairs of the part's packed items
+            bool packed_lines = false;  // some packed query is longer than a strip
         };
         std::vector<PartItems> pitems(parts.size());
         // The largest sxx entry bounds a member's scores: K grows by
@@ -449,12 +450,15 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
         // cell: in the shifted domain a strip's cells lie in [0, max(sxx) *
         // rows].
         // Packed items run in a launch of their own: a plan too small to give
-        // every resident wave items of 16 targets keeps its single launch.
-        const bool packing = opt.packed_cells > 0 || (opt.packed_cells == 0 && item_targets == 16);
+        // every resident wave four items of 8 targets keeps its single launch.
+        const bool packing = opt.packed_cells > 0 || (opt.packed_cells == 0 && item_targets >= 8);
+        // A query longer than one strip is packed when its rebased strips'
+        // cells fit (at most kPackedSpan x max(sxx) above their base).
         auto packable = [&](int32_t lq, int cfg) {
             const TileConfig tc = kTileConfig[cfg];
-            return packing && cfg == 0 && !stackable(lq, cfg) && lq >= kPackedMinRows &&
-                   lq <= tc.lanes * tc.rows && (int64_t)max_sxx * lq <= kPackedMaxScore;
+            if (!packing || cfg != 0 || lq < kPackedMinRows) return false;
+            if (lq <= tc.lanes * tc.rows) return !stackable(lq, cfg) && (int64_t)max_sxx * lq <= kPackedMaxScore;
+            return (int64_t)max_sxx * kPackedSpan <= kPackedMaxScore;
         };
         auto item_cost = [&](int64_t rows_in, const TileConfig& tc, const int2* tg, int32_t cnt) {
             const int groups = 64 / tc.lanes;
@@ -583,6 +587,7 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
                 out.need_lines = out.need_lines || lq > tc.lanes * tc.rows;
                 const bool pk = packable(lq, cfg);
                 if (pk) out.packed += (int64_t)(e - k);
+                if (pk && lq > tc.lanes * tc.rows) out.packed_lines = true;
                 const int32_t flags = (fast[k].query & 1) | ((pk ? kPackedQueue : cfg) << 1);
                 const int64_t first = at;
                 for (size_t u = k; u < e; ++u) {
@@ -651,6 +656,7 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
             need_lines = need_lines || pi.need_lines;
             used |= pi.used;
             hp.info.packed_pairs += pi.packed;
+            hp.packed_lines = hp.packed_lines || pi.packed_lines;
         }
         // One queue per tile configuration and the packed queue (each has its
         // own kernel), every queue costliest first.  A stable sort, so the order is unique: the
@@ -862,7 +868,7 @@ ScratchNeed scratch_need(const HostPlan& hp)
         // The configurations' launches run one after another on the stream
         // and share the lines.
         const int grid = *std::max_element(hp.batch_grid, hp.batch_grid + kBatchQueues);
-        n.batch_lines = up64((size_t)grid * batched_groups_per_block() * hp.batch_stride);
+        n.batch_lines = up64((size_t)grid * batched_groups_per_block() * hp.batch_stride * (hp.packed_lines ? 2 : 1));
     }
     if (!hp.long_pairs.empty()) {
         n.long_lines = up64((size_t)2 * hp.long_nlines * hp.long_stride);

@@ -30,15 +30,17 @@ struct TileConfig {
 };
 constexpr int kTileConfigCount = 3;
 constexpr TileConfig kTileConfig[kTileConfigCount] = {{16, 32}, {16, 16}, {32, 16}};
-// Batched queues: one per tile configuration, and one more for the queries of
-// the 16 x 32 shape that fill one strip on their own and are not stacked,
-// when every score of their pairs fits 15 bits: their kernel scores two
-// targets per lane group in packed 16-bit cells.  An item's queue travels in
-// bits 1..2 of its flags.
+// Batched queues: one per tile configuration, and one more for queries of the
+// 16 x 32 shape whose cells fit the packed kernel's 16-bit cells (those that
+// fill one strip on their own and are not stacked, when max(sxx) x rows <=
+// 0x7BFF; longer ones when max(sxx) x kPackedSpan <= 0x7BFF): that kernel
+// scores two targets per lane group.  An item's queue travels in bits 1..2 of
+// its flags.
 constexpr int kPackedQueue = kTileConfigCount;
 constexpr int kBatchQueues = kTileConfigCount + 1;
 constexpr int32_t kPackedMaxScore = 0x7BFF;   // largest packed cell value
 constexpr int kPackedMinRows = 225;            // packed queries: first-strip heights of 16..32 rows per lane
+constexpr int kPackedSpan = 592;               // a rebased strip's cells span at most kPackedSpan x max(sxx)
 constexpr int kMaxTileBytes = 512;     // largest lanes*rows: one code's profile block
 constexpr int kRowStep = 2;            // rows-per-lane granularity of a query's first strip
 // Stacked items (bit 3 of an item's flags): item.x is the index, in the
@@ -174,6 +176,7 @@ struct HostPlan {
     int64_t batch_first[kBatchQueues + 1] = {};   // items of queue c: [first[c], first[c+1])
     int batch_grid[kBatchQueues] = {};
     int32_t batch_stride = 0;
+    bool packed_lines = false;            // the packed queue has queries longer than a strip (two lines per group)
     // long kernel
     std::vector<int4> long_pairs;
     std::vector<int4> long_tasks;         // {pair, strip, line read, line written} per strip

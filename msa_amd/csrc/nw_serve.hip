@@ -465,8 +465,9 @@ int msa_srv_open(msa_server* srv, const msa_srv_config* cfg)
         const size_t L = (size_t)s->cfg.max_length;
         const size_t stride = (L + 63) & ~size_t(63);
         msa::ScratchNeed n;
+        // (two lines per lane group when a job's plan may take packed cells)
         n.batch_lines = (size_t)per_cu * dev.compute_units * msa::batched_groups_per_block() *
-                        (size_t)msa::batched_line_stride((int64_t)L);
+                        (size_t)msa::batched_line_stride((int64_t)L) * (s->cfg.options.packed_cells < 0 ? 1 : 2);
         n.plain_lines = (size_t)msa::plain_grid(int64_t(1) << 40, dev.compute_units) * stride;
         // Long kernel, only when a job of max_length residues can route a pair
         // there: the planner's line ring for the largest grid, lines of 64-bit

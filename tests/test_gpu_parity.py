@@ -750,3 +750,24 @@ def test_packed_cells_in_the_uneven_set():
         out.close()
         packed.close()
         flat.close()
+
+
+def test_packed_cells_longer_than_a_strip_at_the_top_of_their_range():
+    """Queries longer than one strip take packed cells when their rebased
+    strips' cells fit: max(sxx) x 592 <= 0x7BFF.  All-W sequences of 2000 and
+    3000 residues with max(sxx) = 53 (blosum62, gap 21) make every cell grow
+    as fast as it can, so each strip's base is raised many times close to the
+    bound; with max(sxx) = 54 they stay on int32 cells.  Scores equal the
+    oracle's and the int32 plan's either way."""
+    T, _ = H.tables()["blosum62"]
+    w = W.letter_code("W")
+    rng = np.random.default_rng(9)
+    seqs = [np.full(2000, w, np.uint8), np.full(3000, w, np.uint8), np.full(3000, w, np.uint8)]
+    seqs += [rng.integers(0, 24, int(n)).astype(np.uint8) for n in (700, 1300, 2600)]
+    db = W.pack(seqs)
+    for p, packed in ((21, True), (22, False)):
+        got = assert_oracle(db, T, p, packed=True)
+        assert int(got[2]) == 3000 * 11           # pair (2, 1): identical all-W 3000-residue sequences
+        assert np.array_equal(got, gpu_scores(db, T, p, packed=False))
+        info = plan_info(db, T, p, packed=True)
+        assert (info.packed_pairs > 0) == packed
