@@ -12,9 +12,13 @@ of pairs is checked against the CPU oracle.
 
 Prints one JSON line (rank 0, one GPU).  GCUPS = sum over pairs of the
 unpadded L1*L2 cells / seconds.  The roofline object prices the batched
-execution against the gfx950 int32 VALU peak (256 CU x 64 lanes x 2.4 GHz):
-each cell needs at least 2 int32 VALU operations (one add, one max3) in the
-shifted recurrence the kernels run.
+execution against the gfx950 VALU peak (256 CU x 64 lanes x 2.4 GHz): a cell
+of the shifted recurrence needs at least 2 int32 VALU operations (one add,
+one max3), a packed 16-bit cell 1.5 (one byte permute, one packed add and
+one packed max3 for two cells of two targets), a duo cell 1 (one packed add
+and one packed max3 for two cells of two queries: no permute), so the floor
+is 2 x int32 cells + 1.5 x packed cells + 1 x duo cells lane-operations per
+execution.  `int32_equivalent_frac` prices every cell at 2.
 
 Usage: python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--workload uneven]
 """
@@ -34,6 +38,8 @@ sys.path.insert(0, ROOT)
 METRIC = "all-pairs NW GCUPS on 1 MI355X (batched vs plain, bit-exact) + serving p50/p99 latency"
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12     # int32 lane-ops per second, in T
 OPS_PER_CELL = 2
+PACKED_OPS_PER_CELL = 1.5
+DUO_OPS_PER_CELL = 1.0
 DEFAULT_TUNING = os.path.join(ROOT, "msa_amd", "tuning", "gfx950.json")
 
 
@@ -312,20 +318,32 @@ def measure(N, w, steps, warmup, plain_steps, oracle_pairs, stream, out, tuning,
     return batched, plain, same, oracle_ok, oracle_n
 
 
+def valu_roofline(w, info, ms):
+    """Roofline of one batched execution: the floor of VALU lane-operations
+    (2 per int32 cell, 1.5 per packed cell, 1 per duo cell) over its time,
+    against the peak."""
+    cells = w.cells()
+    packed = int(getattr(info, "packed_cells", 0))
+    duo = int(getattr(info, "duo_cells", 0))
+    ops = OPS_PER_CELL * (cells - packed) + PACKED_OPS_PER_CELL * (packed - duo) + DUO_OPS_PER_CELL * duo
+    achieved = ops / (ms * 1e-3) / 1e12
+    kernel = dominant_kernel(info)
+    return {"bound": "valu", "kernel": kernel, "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2),
+            "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": measured_traffic(w.name, kernel),
+            "ops_per_cell": round(ops / cells, 4), "packed_cells": packed, "duo_cells": duo,
+            "int32_equivalent_frac": round(OPS_PER_CELL * cells / (ms * 1e-3) / 1e12 / VALU_PEAK_TOPS, 4)}
+
+
 def config_record(w, batched, plain, same, oracle_ok, oracle_n):
     cells = w.cells()
     info = batched["info"]
-    achieved = OPS_PER_CELL * cells / (batched["ms"] * 1e-3) / 1e12
-    kernel = dominant_kernel(info)
     return {"description": w.description, "pairs": w.npairs, "cells": cells, "input_checksum": w.checksum(),
             "gcups": round(cells / batched["ms"] / 1e6, 2), "ms_per_step": round(batched["ms"], 3),
             "plain_gcups": round(cells / plain["ms"] / 1e6, 2),
             "auto_vs_plain": round(plain["ms"] / batched["ms"], 3),
             "routing": {"batched_pairs": int(info.batched_pairs), "long_pairs": int(info.long_pairs),
                         "plain_pairs": int(info.plain_pairs)},
-            "roofline": {"bound": "valu", "kernel": kernel, "achieved": round(achieved, 3),
-                         "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s",
-                         "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": measured_traffic(w.name, kernel)},
+            "roofline": valu_roofline(w, info, batched["ms"]),
             "bit_exact": {"auto_equals_plain_all_pairs": same, "oracle_sample_pairs": oracle_n,
                           "oracle_sample_equal": oracle_ok}}
 
@@ -556,7 +574,6 @@ def main():
                                                         args.oracle_pairs, stream, out, tuning)
 
     gcups = cells / batched["ms"] / 1e6
-    achieved = OPS_PER_CELL * cells / (batched["ms"] * 1e-3) / 1e12
     info = batched["info"]
     result = {
         "metric": METRIC,
@@ -577,6 +594,8 @@ def main():
             "launches_per_step": int(info.launches), "bins": int(info.bins),
             "batched_pairs": int(info.batched_pairs), "long_pairs": int(info.long_pairs),
             "plain_routed_pairs": int(info.plain_pairs),
+            "stacked_pairs": int(info.stacked_pairs), "packed_pairs": int(info.packed_pairs),
+            "duo_pairs": int(info.duo_pairs),
             "graph": True, "parallelism": "single GPU",
             "tuning": tuning_summary(tuning),
         },
@@ -586,10 +605,7 @@ def main():
         "bit_exact": {"batched_equals_plain_all_pairs": same, "oracle_sample_pairs": oracle_n,
                       "oracle_sample_equal": oracle_ok},
         "wall_ms_per_step": round(batched["wall_ms"], 3),
-        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2),
-                     "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
-                     "traffic": measured_traffic(w.name, dominant_kernel(info)),
-                     "kernel": dominant_kernel(info), "ops_per_cell": OPS_PER_CELL},
+        "roofline": valu_roofline(w, info, batched["ms"]),
     }
     # Host in, host out (msa_nw_align): planning, one upload, the kernels and
     # the score download, once -- the PCIe-inclusive rate, never `value`.

@@ -153,6 +153,9 @@ typedef struct {
                                 rebased as they go): 0 (default) in plans large enough to give
                                 every resident wave four items of 8 targets (a small plan keeps its
                                 single launch); 1 in every plan; -1 never (every cell an int32) */
+    int duo_queries;         /* packed queries are scored two at a time against each target (one
+                                query per half of a packed cell) where their target lists agree:
+                                0 (default) yes, -1 no (two targets per lane group instead) */
 } msa_nw_options;
 
 typedef struct {
@@ -166,6 +169,9 @@ typedef struct {
     int64_t device_bytes;    /* device memory the plan holds */
     int64_t stacked_pairs;   /* batched pairs scored in stacked strips (several queries to a strip) */
     int64_t packed_pairs;    /* batched pairs scored in packed 16-bit cells (two targets per lane) */
+    int64_t packed_cells;    /* sum over those pairs of unpadded L1 * L2 */
+    int64_t duo_pairs;       /* packed pairs scored two queries to a lane group */
+    int64_t duo_cells;       /* sum over those pairs of unpadded L1 * L2 */
 } msa_nw_plan_info;
 
 typedef struct msa_nw_plan_opaque* msa_nw_plan;

@@ -54,13 +54,15 @@ class NwOptions(ctypes.Structure):
     _fields_ = [("kernel", ctypes.c_int), ("use_graph", ctypes.c_int),
                 ("lanes_per_pair", ctypes.c_int), ("rows_per_lane", ctypes.c_int),
                 ("long_threshold", ctypes.c_int), ("tuning_file", ctypes.c_char_p),
-                ("single_query_strips", ctypes.c_int), ("packed_cells", ctypes.c_int)]
+                ("single_query_strips", ctypes.c_int), ("packed_cells", ctypes.c_int),
+                ("duo_queries", ctypes.c_int)]
 
 
 class PlanInfo(ctypes.Structure):
     _fields_ = [("pairs", _i64), ("cells", _i64), ("plain_pairs", _i64), ("batched_pairs", _i64),
                 ("long_pairs", _i64), ("launches", _i32), ("bins", _i32), ("device_bytes", _i64),
-                ("stacked_pairs", _i64), ("packed_pairs", _i64)]
+                ("stacked_pairs", _i64), ("packed_pairs", _i64),
+                ("packed_cells", _i64), ("duo_pairs", _i64), ("duo_cells", _i64)]
 
 
 class SrvConfig(ctypes.Structure):
@@ -254,12 +256,13 @@ class DeviceBuffer:
 
 
 def options(kernel=KERNEL_AUTO, use_graph=False, lanes=0, rows=0, long_threshold=0, tuning_file=None, stack=True,
-            packed=None):
+            packed=None, duo=True):
     """packed: None = packed cells in large plans (the library's default),
-    True = in every plan, False = never."""
+    True = in every plan, False = never.  duo: packed queries two to a lane
+    group where their target lists agree (the default), else two targets."""
     path = os.fsencode(tuning_file) if tuning_file else None
     return NwOptions(kernel, 1 if use_graph else 0, lanes, rows, long_threshold, path, 0 if stack else 1,
-                     0 if packed is None else (1 if packed else -1))
+                     0 if packed is None else (1 if packed else -1), 0 if duo else -1)
 
 
 class Plan:

@@ -23,7 +23,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MSA_OFFLOAD_ARCH", "gfx950")
 
 # Longest compiles first (the batched kernel's three tile configurations).
-HIP_SOURCES = ["nw_batched.hip", "nw_batched_32x16.hip", "nw_batched_16x16.hip", "nw_batched_packed.hip",
+HIP_SOURCES = ["nw_batched.hip", "nw_batched_32x16.hip", "nw_batched_16x16.hip", "nw_batched_packed.hip", "nw_batched_duo.hip",
                "runtime.hip", "nw_plain.hip",
                "nw_long.hip", "nw_plan.hip", "nw_noalloc.hip", "nw_serve.hip", "nw_tune.hip", "nj.hip", "pgalign.hip",
                "fasta.hip"]
@@ -40,7 +40,7 @@ def _run(cmd, cwd=None):
 # The pairwise kernels' sources: their compiles also report each kernel's
 # registers, LDS and occupancy (the compiler's kernel-resource-usage remarks),
 # kept in kernel_resources.json next to the library for the tuner's record.
-RESOURCE_SOURCES = ("nw_batched.hip", "nw_batched_32x16.hip", "nw_batched_16x16.hip", "nw_batched_packed.hip",
+RESOURCE_SOURCES = ("nw_batched.hip", "nw_batched_32x16.hip", "nw_batched_16x16.hip", "nw_batched_packed.hip", "nw_batched_duo.hip",
                     "nw_plain.hip",
                     "nw_long.hip")
 RESOURCE_FILE = os.path.join(PKG, "kernel_resources.json")

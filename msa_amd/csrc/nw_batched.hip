@@ -30,14 +30,15 @@ int batched_groups_per_block() { return batched::kWaves * batched::kMaxGroups; }
 
 int batched_occupancy(int* blocks_per_cu)
 {
-    // The smallest of the four kernels' occupancies: a plan sizes its grids
+    // The smallest of the five kernels' occupancies: a plan sizes its grids
     // and line buffers once for whichever configurations it uses.
     int n[kBatchQueues] = {};
     MSA_TRY(batched::occupancy_16x32(&n[0]));
     MSA_TRY(batched::occupancy_16x16(&n[1]));
     MSA_TRY(batched::occupancy_32x16(&n[2]));
     MSA_TRY(batched::occupancy_packed(&n[3]));
-    *blocks_per_cu = std::min({n[0], n[1], n[2], n[3]});
+    MSA_TRY(batched::occupancy_duo(&n[4]));
+    *blocks_per_cu = std::min({n[0], n[1], n[2], n[3], n[4]});
     return MSA_OK;
 }
 
@@ -67,7 +68,8 @@ int batched_launch(int config, int grid, const uint16_t* d_blocks, const SeqInfo
     case 0: return batched::launch_16x32(grid, a, stream);
     case 1: return batched::launch_16x16(grid, a, stream);
     case 2: return batched::launch_32x16(grid, a, stream);
-    default: return batched::launch_packed(grid, a, stream);
+    case 3: return batched::launch_packed(grid, a, stream);
+    default: return batched::launch_duo(grid, a, stream);
     }
 }
 

@@ -71,18 +71,24 @@ constexpr int kTabCopy = kCodes * kTabStride;                  // 700 bytes
 constexpr int kTableBytes = (2 * kTabCopy + 15) / 16 * 16;     // both copies, padded to 16 bytes
 // One wave's code rings: per lane group 4G bytes (one ring of 2G columns,
 // twice), 256 bytes a wave; in the packed kernel 6G bytes (two rings of 3G
-// bytes, one per target), 384 bytes a wave of four 16-lane groups.
+// bytes, one per target), 384 bytes a wave of four 16-lane groups (the duo
+// kernel uses the first ring of each group).
 constexpr int kRingBytes = 256;
 constexpr int kPackedRingBytes = 384;
+constexpr int kItemSlotBytes = 16;   // duo kernel: the workgroup's current item
 
-// Dynamic LDS of a workgroup: the waves' profile regions, the table, the rings.
-constexpr size_t lds_bytes(bool packed = false)
+constexpr bool packed_queue(int cfg) { return cfg == kPackedQueue || cfg == kDuoQueue; }
+
+// Dynamic LDS of a workgroup: the waves' profile regions (in the duo kernel
+// one shared profile), the table, the rings, the duo kernel's item slot.
+constexpr size_t lds_bytes(int cfg = 0)
 {
-    return (size_t)kWaves * kProfWords * 4 + kTableBytes + kWaves * (packed ? kPackedRingBytes : kRingBytes);
+    return (size_t)kWaves * kProfWords * 4 + kTableBytes + kWaves * (packed_queue(cfg) ? kPackedRingBytes : kRingBytes) +
+           (cfg == kDuoQueue ? kItemSlotBytes : 0);
 }
 
 static_assert(kTileConfigCount == 3, "one kernel source per tile configuration");
-static_assert(kPackedQueue == 3 && kBatchQueues == 4, "the packed kernel is the fourth batched queue");
+static_assert(kPackedQueue == 3 && kDuoQueue == 4 && kBatchQueues == 5, "the packed kernels are the last two queues");
 
 struct BatchArgs {
     const uint16_t* blocks;
@@ -631,15 +637,19 @@ __device__ __forceinline__ void cells_packed(uint32_t (&h)[R], uint32_t up, uint
                                              const uint32_t (&wa)[dev::prof_words<R>],
                                              const uint32_t (&wb)[dev::prof_words<R>])
 {
-    uint32_t d = diag;
+    // bytes: A's byte r%4, 0, B's byte r%4, 0
+    constexpr uint32_t kSel[4] = {0x0C040C00u, 0x0C050C01u, 0x0C060C02u, 0x0C070C03u};
     uint32_t u = up;
+    // Row r+1's permute comes before row r's max3 and its add after: no
+    // packed add then reads the instruction just before it, which would cost
+    // a wait state before the max3 that reads the add.
+    uint32_t t = pk_add(diag, __builtin_amdgcn_perm(wb[0], wa[0], kSel[0]));
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        // bytes: A's byte r%4, 0, B's byte r%4, 0
-        constexpr uint32_t kSel[4] = {0x0C040C00u, 0x0C050C01u, 0x0C060C02u, 0x0C070C03u};
-        const uint32_t s = __builtin_amdgcn_perm(wb[r >> 2], wa[r >> 2], kSel[r & 3]);
-        const uint32_t nh = pk_max3(pk_add(d, s), u, h[r]);
-        d = h[r];
+        uint32_t s = 0;
+        if (r + 1 < R) s = __builtin_amdgcn_perm(wb[(r + 1) >> 2], wa[(r + 1) >> 2], kSel[(r + 1) & 3]);
+        const uint32_t nh = pk_max3(t, u, h[r]);
+        if (r + 1 < R) t = pk_add(h[r], s);
         h[r] = nh;
         u = nh;
     }
@@ -909,6 +919,305 @@ __device__ __forceinline__ void run_item_packed(const BatchArgs& a, const int4 i
     }
 }
 
+// ---- Duo items: two queries per lane group ------------------------------
+//
+// A duo item pairs two queries of the packed kernel's kind whose target lists
+// (almost) agree, and each lane group scores one target against both: the
+// low half of every packed cell belongs to the first query, the high half to
+// the second.  Both queries share the column, so one profile word per (column
+// code, row) holds both queries' sxx -- the cell update is one packed add and
+// one packed max3 per two cells, with no byte permute.  That profile takes
+// four bytes a row, 51,200 bytes for a 16 x 32 strip: the four waves of a
+// workgroup share it and take whole items together, each lane group one
+// target of the item's round of 16.  The queries are aligned at their last
+// rows (the shorter one gets neutral rows on top); lines, bases and results
+// work per half as in the packed kernel.
+
+// Profile layout of a duo strip of R rows per lane: per column code, planes of
+// 16 bytes per lane ([plane][lane][4 words]), word r of a lane holding row r
+// of both queries; a lane reads its words of a code at its slot plus constants.
+template <int R>
+constexpr int duo_planes = (R + 3) / 4;
+template <int R>
+constexpr int duo_code_words = 16 * 4 * duo_planes<R>;
+static_assert(kCodes * duo_code_words<32> == kWaves * kProfWords, "a duo profile fills the waves' profile regions");
+
+// Words [B, E) of a lane's profile words of one code (B a multiple of 4, E even).
+template <int R, int B = 0, int E = R>
+__device__ __forceinline__ void load_prof_duo(uint32_t (&w)[R], const dev::lds_u32* code_base, int k)
+{
+    static_assert(R % 2 == 0 && B % 4 == 0 && E % 2 == 0 && E <= R, "strip heights are even");
+    const dev::lds_u32* slot = code_base + k * 4;
+#pragma unroll
+    for (int j = B / 4; j < E / 4; ++j) {
+        const dev::u32x4 v = *(const dev::lds_u32x4*)(slot + j * 64);
+        w[4 * j + 0] = v.x;
+        w[4 * j + 1] = v.y;
+        w[4 * j + 2] = v.z;
+        w[4 * j + 3] = v.w;
+    }
+    if constexpr (E % 4 == 2) {
+        const dev::u32x2 v = *(const dev::lds_u32x2*)(slot + (E / 4) * 64);
+        w[E - 2] = v.x;
+        w[E - 1] = v.y;
+    }
+}
+
+// Builds a duo strip's profile with the whole workgroup: rows [row0, row0 +
+// 16R) of the padded row space, where query q's row i sits at i + pad_q (rows
+// of a query before its first are neutral; an absent second query, q2 ==
+// nullptr, is neutral throughout).  Per (lane, row) slot, four-code runs of
+// both queries' table rows become four words by byte permutes (word of code c:
+// bytes q1's sxx, 0, q2's sxx, 0).
+__device__ __forceinline__ void build_profile_duo(dev::lds_u32* prof, const dev::lds_u8* tab, const uint16_t* q1,
+                                                  const uint16_t* q2, int row0, int pad1, int pad2, bool tr1,
+                                                  bool tr2, int R, int tid)
+{
+    constexpr uint32_t kSel[4] = {0x0C040C00u, 0x0C050C01u, 0x0C060C02u, 0x0C070C03u};
+    const int code_words_r = 16 * 4 * ((R + 3) / 4);
+    for (int slot = tid; slot < 16 * R; slot += kWaves * 64) {
+        const int k = slot % 16;
+        const int r = slot / 16;
+        const int p = row0 + k * R + r;
+        const int i1 = p - pad1, i2 = p - pad2;
+        const int c1 = i1 >= 0 ? min(dev::code_at(q1, i1), kNull) : kNull;
+        const int c2 = (q2 != nullptr && i2 >= 0) ? min(dev::code_at(q2, i2), kNull) : kNull;
+        const dev::lds_u8* t1 = tab + (tr1 ? kTabCopy : 0) + c1 * kTabStride;
+        const dev::lds_u8* t2 = tab + (tr2 ? kTabCopy : 0) + c2 * kTabStride;
+        dev::lds_u32* dst = prof + (r >> 2) * 64 + k * 4 + (r & 3);
+#pragma unroll
+        for (int j = 0; j < (kCodes + 3) / 4; ++j) {
+            const uint32_t a = *(const dev::lds_u32*)(t1 + 4 * j);
+            const uint32_t b = *(const dev::lds_u32*)(t2 + 4 * j);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (4 * j + i < kCodes) dst[(4 * j + i) * code_words_r] = __builtin_amdgcn_perm(b, a, kSel[i]);
+        }
+    }
+}
+
+// Rows [B, E) of one step of a lane's R rows, two queries at once (h packed;
+// u: the cell above row B, new; t: row B's diagonal plus its sxx; w: the
+// column code's profile words).  Row r+1's add comes after row r's max3.
+template <int R, int B, int E>
+__device__ __forceinline__ void cells_duo(uint32_t (&h)[R], uint32_t& u, uint32_t& t, const uint32_t (&w)[R])
+{
+    // Rows in runs of four: the run's adds first (row B's is `t`), then its
+    // chain of max3s.
+#pragma unroll
+    for (int r0 = B; r0 < E; r0 += 4) {
+        constexpr int N = 4;
+        uint32_t s[N];
+        s[0] = t;
+#pragma unroll
+        for (int i = 1; i < N; ++i)
+            if (r0 + i < E) s[i] = pk_add(h[r0 + i - 1], w[r0 + i]);
+        if (r0 + N < R && r0 + N <= E) t = pk_add(h[r0 + N - 1], w[r0 + N]);
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            if (r0 + i < E) {
+                const uint32_t nh = pk_max3(s[i], u, h[r0 + i]);
+                h[r0 + i] = nh;
+                u = nh;
+            }
+    }
+}
+
+// One duo strip of R rows per lane over the target `t`: run_strip_packed with
+// one target, one code ring (3G bytes) and one profile read per step; the
+// lines `pl.la` and `pl.lb` are the first and the second query's.
+template <int R, bool LOAD, bool STORE>
+__device__ __forceinline__ uint32_t run_strip_duo(const dev::lds_u32* prof, const TargetState& t,
+                                                  volatile dev::lds_u8* ring, const PackedLines& pl, int wchunks,
+                                                  int k, int margin, int& base_a, int& base_b)
+{
+    constexpr int G = 16;
+    constexpr int CODE_WORDS = duo_code_words<R>;
+    constexpr int RING = 2 * G;
+    uint32_t h[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) h[r] = 0;
+    uint32_t dtop = 0;
+    base_a = 0;
+    base_b = 0;
+
+    ring[G + k] = kNull;
+    {
+        const int c = col_code<G>(t, 0, k);
+        ring[k] = c;
+        ring[RING + k] = c;
+    }
+    int nc = col_code<G>(t, 1, k);
+    ColumnCursor<G> cur(t, 2, k);
+    uint32_t ln = 0;
+    int la_next = -1, lb_next = -1;
+    if constexpr (LOAD) {
+        ln = packed_rel(packed_line_abs(t, pl.la, 0, k), 0, packed_line_abs(t, pl.lb, 0, k), 0);
+        la_next = packed_line_abs(t, pl.la, 1, k);
+        lb_next = packed_line_abs(t, pl.lb, 1, k);
+    }
+
+    const int base0 = (RING - k) & (RING - 1);
+    int bn = ring[base0 + 1];
+    uint32_t w[R];
+    load_prof_duo<R>(w, prof + ring[base0] * CODE_WORDS, k);
+    // Tall strips read the next column's words in two halves, each after the
+    // current column's half is done (16 registers of read-ahead, not R).
+    constexpr int H = R >= 20 ? (R / 2) / 4 * 4 : 0;
+
+    for (int c = 0; c < wchunks; ++c) {
+        const int base = (c * G + 2 - k) & (RING - 1);
+        const uint32_t va = pl.off_a + (uint32_t)(c * G - (G - 1) - t.lpad) * 4u;
+        const uint32_t vb = pl.off_b + (uint32_t)(c * G - (G - 1) - t.lpad) * 4u;
+        static_for<0, G>([&](auto uc) {
+            constexpr int u = decltype(uc)::value;
+            if constexpr (u == G - 3) {
+                const int slot = ((c + 1) * G + k) & (RING - 1);
+                ring[slot] = nc;
+                if (slot < G) ring[RING + slot] = nc;
+            }
+            const int bnn = ring[base + u];
+            uint32_t hup;
+            if constexpr (LOAD)
+                hup = (uint32_t)dev::dpp<dev::kRowShr1>(row_bcast<u>((int)ln), (int)h[R - 1]);
+            else
+                hup = (uint32_t)dev::dpp_zero<dev::kRowShr1>((int)h[R - 1]);
+            uint32_t up = hup;
+            uint32_t tt = pk_add(dtop, w[0]);
+            if constexpr (H > 0) {
+                cells_duo<R, 0, H>(h, up, tt, w);
+                load_prof_duo<R, 0, H>(w, prof + bn * CODE_WORDS, k);
+                cells_duo<R, H, R>(h, up, tt, w);
+                load_prof_duo<R, H, R>(w, prof + bn * CODE_WORDS, k);
+            } else {
+                uint32_t wn[R];
+                load_prof_duo<R>(wn, prof + bn * CODE_WORDS, k);
+                cells_duo<R, 0, R>(h, up, tt, w);
+#pragma unroll
+                for (int q = 0; q < R; ++q) w[q] = wn[q];
+            }
+            dtop = hup;
+            if constexpr (STORE) {
+                const int va_abs = (int)(h[R - 1] & 0xFFFFu) + base_a;
+                const int vb_abs = (int)(h[R - 1] >> 16) + base_b;
+                __builtin_amdgcn_raw_buffer_store_b32((uint32_t)va_abs, pl.rsrc, (int)va, 4 * u, 0);
+                __builtin_amdgcn_raw_buffer_store_b32((uint32_t)vb_abs, pl.rsrc, (int)vb, 4 * u, 0);
+            }
+            bn = bnn;
+        });
+        if constexpr (LOAD) {
+            if ((c & 3) == 3) {
+                const int fa = row_bcast<0>(la_next), fb = row_bcast<0>(lb_next);
+                const int nba = fa < 0 ? base_a : max(base_a, fa - margin);
+                const int nbb = fb < 0 ? base_b : max(base_b, fb - margin);
+                const uint32_t delta = (uint32_t)(nba - base_a) | ((uint32_t)(nbb - base_b) << 16);
+#pragma unroll
+                for (int r = 0; r < R; ++r) h[r] = pk_sub(h[r], delta);
+                dtop = pk_sub(dtop, delta);
+                base_a = nba;
+                base_b = nbb;
+            }
+            ln = packed_rel(la_next, base_a, lb_next, base_b);
+            la_next = packed_line_abs(t, pl.la, c + 2, k);
+            lb_next = packed_line_abs(t, pl.lb, c + 2, k);
+        }
+        nc = cur.code(t);
+        cur.advance();
+    }
+    return h[R - 1];
+}
+
+template <bool STORE, int R = kRowStep>
+__device__ __forceinline__ uint32_t run_first_strip_duo(int r0, const dev::lds_u32* prof, const TargetState& t,
+                                                        volatile dev::lds_u8* ring, const PackedLines& pl, int wchunks,
+                                                        int k, int margin, int& base_a, int& base_b)
+{
+    if constexpr (R >= 32) {
+        return run_strip_duo<32, false, STORE>(prof, t, ring, pl, wchunks, k, margin, base_a, base_b);
+    } else {
+        if (r0 == R) return run_strip_duo<R, false, STORE>(prof, t, ring, pl, wchunks, k, margin, base_a, base_b);
+        return run_first_strip_duo<STORE, R + kRowStep>(r0, prof, t, ring, pl, wchunks, k, margin, base_a, base_b);
+    }
+}
+
+// One duo item with the whole workgroup: rounds of 16 targets (one per lane
+// group), strip after strip; the shared profile is built once for a pair of
+// one-strip queries, else per strip and round, between workgroup barriers.
+__device__ __forceinline__ void run_item_duo(const BatchArgs& a, const int4 item, dev::lds_u32* prof,
+                                             const dev::lds_u8* tab, dev::lds_u8* rings, int32_t* lines, int wave,
+                                             int lane)
+{
+    constexpr int G = 16, RMAX = 32, GROUPS = 4, STRIP_MAX = G * RMAX, SLOTS = kWaves * GROUPS;
+    const int g = lane / G;
+    const int k = lane % G;
+    volatile dev::lds_u8* ring = rings + g * 3 * G;
+    const int2 rec = a.targets[item.y];
+    const SeqInfo q1i = a.info[item.x];
+    const int lq1 = q1i.z;
+    const uint16_t* q1 = a.blocks + q1i.x;
+    int lq2 = 0;
+    const uint16_t* q2 = nullptr;
+    if (rec.x >= 0) {
+        const SeqInfo q2i = a.info[rec.x];
+        lq2 = q2i.z;
+        q2 = a.blocks + q2i.x;
+    }
+    const int Lq = max(lq1, lq2);
+    const int full = (Lq - 1) / STRIP_MAX;
+    const int rem = Lq - full * STRIP_MAX;
+    const int R0 = min(RMAX, ((rem + G * kRowStep - 1) / (G * kRowStep)) * kRowStep);
+    const int pad_top = G * R0 - rem;
+    const int pad1 = pad_top + Lq - lq1, pad2 = pad_top + Lq - lq2;
+    const int nst = full + 1;
+    PackedLines pl;
+    {
+        pl.la = lines + (int64_t)(2 * g) * a.line_stride + kLineGuard;
+        pl.lb = pl.la + a.line_stride;
+        const uint64_t wb = (uint64_t)(uintptr_t)lines;
+        const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)wb);
+        const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(wb >> 32));
+        pl.rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)(((uint64_t)hi << 32) | lo), 0,
+                                                    2 * kMaxGroups * a.line_stride * 4, kRawBufferFlags);
+        pl.off_a = k == G - 1 ? (uint32_t)(2 * g * a.line_stride + kLineGuard) * 4u : kOutOfRange;
+        pl.off_b = k == G - 1 ? (uint32_t)((2 * g + 1) * a.line_stride + kLineGuard) * 4u : kOutOfRange;
+    }
+    const int rounds = (item.z + SLOTS - 1) / SLOTS;
+    for (int rd = 0; rd < rounds; ++rd) {
+        const int ti = rd * SLOTS + wave * GROUPS + g;
+        const bool valid = ti < item.z;
+        const int2 tg = valid ? a.targets[item.y + 1 + 2 * ti] : make_int2(0, -1);
+        const int out2 = valid ? a.targets[item.y + 2 + 2 * ti].y : -1;
+        const TargetState t = target_state(a, tg, valid);
+        int wchunks = t.chunks;
+        wchunks = max(wchunks, __shfl_xor(wchunks, 16));
+        wchunks = max(wchunks, __shfl_xor(wchunks, 32));
+        uint32_t res = 0;
+        int base_a = 0, base_b = 0;
+        for (int st = 0; st < nst; ++st) {
+            if (nst > 1 || rd == 0) {
+                __syncthreads();
+                const int row0 = st == 0 ? 0 : G * R0 + (st - 1) * STRIP_MAX;
+                build_profile_duo(prof, tab, q1, q2, row0, pad1, pad2, (item.w & 1) != 0, (rec.y & 1) != 0,
+                                  st == 0 ? R0 : RMAX, wave * 64 + lane);
+                __syncthreads();
+            }
+            const bool store = st + 1 < nst;
+            if (st == 0)
+                res = store ? run_first_strip_duo<true>(R0, prof, t, ring, pl, wchunks, k, a.margin, base_a, base_b)
+                            : run_first_strip_duo<false>(R0, prof, t, ring, pl, wchunks, k, a.margin, base_a, base_b);
+            else
+                res = store ? run_strip_duo<RMAX, true, true>(prof, t, ring, pl, wchunks, k, a.margin, base_a, base_b)
+                            : run_strip_duo<RMAX, true, false>(prof, t, ring, pl, wchunks, k, a.margin, base_a, base_b);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        }
+        if (k == G - 1) {
+            if (tg.y >= 0) a.out[tg.y] = (int)(res & 0xFFFFu) + base_a - (lq1 + t.len) * a.penalty;
+            if (out2 >= 0) a.out[out2] = (int)(res >> 16) + base_b - (lq2 + t.len) * a.penalty;
+        }
+        wave_lds_sync();
+    }
+}
+
 template <int CFG>
 __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(3))) void nw_batched_kernel(BatchArgs a)
 {
@@ -918,7 +1227,7 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(3))
     dev::lds_u32* lds_base = (dev::lds_u32*)lds;   // address-space cast: 32-bit LDS offsets
     dev::lds_u32* prof = lds_base + wave * kProfWords;
     dev::lds_u8* tab = (dev::lds_u8*)(lds_base + kWaves * kProfWords);
-    dev::lds_u8* rings = tab + kTableBytes + wave * (CFG == kPackedQueue ? kPackedRingBytes : kRingBytes);
+    dev::lds_u8* rings = tab + kTableBytes + wave * (packed_queue(CFG) ? kPackedRingBytes : kRingBytes);
     for (int i = threadIdx.x; i < 2 * kTabCopy; i += kWaves * 64) {
         const int flip = i >= kTabCopy;
         const int j = i - flip * kTabCopy;
@@ -927,17 +1236,30 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(3))
     }
     __syncthreads();
 
-    // A wave's line regions: one line per lane group, two in the packed kernel.
-    int32_t* lines = a.lines + (int64_t)(blockIdx.x * kWaves + wave) * (CFG == kPackedQueue ? 2 : 1) * kMaxGroups *
+    // A wave's line regions: one line per lane group, two in the packed kernels.
+    int32_t* lines = a.lines + (int64_t)(blockIdx.x * kWaves + wave) * (packed_queue(CFG) ? 2 : 1) * kMaxGroups *
                                a.line_stride;
 
-    while (true) {
-        int it = 0;
-        if (lane == 0) it = atomicAdd(a.queue, 1);
-        it = __shfl(it, 0);
-        if (it >= a.nitems) break;
-        if constexpr (CFG == kPackedQueue) run_item_packed(a, a.items[it], prof, tab, rings, lines, lane);
-        else run_item<kTileConfig[CFG].lanes, kTileConfig[CFG].rows>(a, a.items[it], prof, tab, rings, lines, lane);
+    if constexpr (CFG == kDuoQueue) {
+        // The workgroup takes items together (they share the profile).
+        volatile dev::lds_u32* slot = (volatile dev::lds_u32*)(tab + kTableBytes + kWaves * kPackedRingBytes);
+        while (true) {
+            if (threadIdx.x == 0) *slot = (uint32_t)atomicAdd(a.queue, 1);
+            __syncthreads();
+            const int it = (int)*slot;
+            __syncthreads();
+            if (it >= a.nitems) break;
+            run_item_duo(a, a.items[it], lds_base, tab, rings, lines, wave, lane);
+        }
+    } else {
+        while (true) {
+            int it = 0;
+            if (lane == 0) it = atomicAdd(a.queue, 1);
+            it = __shfl(it, 0);
+            if (it >= a.nitems) break;
+            if constexpr (CFG == kPackedQueue) run_item_packed(a, a.items[it], prof, tab, rings, lines, lane);
+            else run_item<kTileConfig[CFG].lanes, kTileConfig[CFG].rows>(a, a.items[it], prof, tab, rings, lines, lane);
+        }
     }
 }
 
@@ -946,7 +1268,7 @@ template <int CFG>
 int kernel_occupancy(int* blocks_per_cu)
 {
     const void* fn = reinterpret_cast<const void*>(&nw_batched_kernel<CFG>);
-    const size_t lds = lds_bytes(CFG == kPackedQueue);
+    const size_t lds = lds_bytes(CFG);
     MSA_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int n = 0;
     MSA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kWaves * 64, lds));
@@ -957,7 +1279,7 @@ int kernel_occupancy(int* blocks_per_cu)
 template <int CFG>
 int kernel_launch(int grid, const BatchArgs& a, hipStream_t stream)
 {
-    hipLaunchKernelGGL(nw_batched_kernel<CFG>, dim3(grid), dim3(kWaves * 64), lds_bytes(CFG == kPackedQueue), stream, a);
+    hipLaunchKernelGGL(nw_batched_kernel<CFG>, dim3(grid), dim3(kWaves * 64), lds_bytes(CFG), stream, a);
     return hip_status(hipGetLastError(), "nw_batched_kernel launch");
 }
 
@@ -971,6 +1293,8 @@ int occupancy_32x16(int* blocks_per_cu);
 int launch_32x16(int grid, const BatchArgs& a, hipStream_t stream);
 int occupancy_packed(int* blocks_per_cu);
 int launch_packed(int grid, const BatchArgs& a, hipStream_t stream);
+int occupancy_duo(int* blocks_per_cu);
+int launch_duo(int grid, const BatchArgs& a, hipStream_t stream);
 
 }  // namespace batched
 }  // namespace msa

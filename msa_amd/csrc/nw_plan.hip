@@ -24,6 +24,7 @@
 #include "nw_planner.hpp"
 
 #include <algorithm>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -423,11 +424,13 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
         struct PartItems {
             std::vector<Item> items;
             std::vector<Group> stackable;
+            std::vector<Group> duoable;  // packed query groups that may share duo items
             int64_t kept = 0;           // pairs of the part's groups that are not stacked
             int32_t max_t = 1;
             bool need_lines = false;
             unsigned used = 0;
             int64_t packed = 0;         // pairs of the part's packed items
+            int64_t packed_cells = 0;   // their unpadded cells
             bool packed_lines = false;  // some packed query is longer than a strip
         };
         std::vector<PartItems> pitems(parts.size());
@@ -460,6 +463,10 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
             if (lq <= tc.lanes * tc.rows) return !stackable(lq, cfg) && (int64_t)max_sxx * lq <= kPackedMaxScore;
             return (int64_t)max_sxx * kPackedSpan <= kPackedMaxScore;
         };
+        // Packed queries go in pairs to the duo kernel (two queries against
+        // one target per lane group, about 1.1 VALU operations per cell
+        // against 1.6 in the packed kernel) when their target lists agree.
+        const bool duo = packing && opt.duo_queries >= 0;
         auto item_cost = [&](int64_t rows_in, const TileConfig& tc, const int2* tg, int32_t cnt) {
             const int groups = 64 / tc.lanes;
             const int64_t step = (int64_t)kRowStep * tc.lanes;
@@ -482,6 +489,7 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
                 const int32_t lq = seqs[fast[k].query >> 1].first_end;
                 const int cfg = group_cfg(lq);
                 if (stackable(lq, cfg)) out.stackable.push_back({t, fast[k].query, cfg, (int64_t)k, (int64_t)e});
+                else if (duo && packable(lq, cfg)) out.duoable.push_back({t, fast[k].query, cfg, (int64_t)k, (int64_t)e});
                 else out.kept += (int64_t)(e - k);
                 k = e;
             }
@@ -550,9 +558,83 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
             for (size_t s = (size_t)t; s < stacks.size(); s += (size_t)T) stacks[s].uni = merge(stacks[s], nullptr, nullptr);
         });
         if (!ok) return fail(MSA_ERR_OUT_OF_MEMORY, "host allocation failed while planning");
+        // Duo pairs: the duo candidates by orientation (a query's transposed
+        // and plain groups hold different targets), then in rank order
+        // (length descending), each paired with the next when the pair's padded work (the union
+        // of the two target lists, both queries at the longer one's rows)
+        // stays within 1.3 times the work of their own pairs.  A query left
+        // alone takes duo items of its own (their high halves idle) when such
+        // queries hold at most a tenth of the pairs, else the packed kernel.
+        std::vector<Group> dcand;
+        for (const PartItems& pi : pitems) dcand.insert(dcand.end(), pi.duoable.begin(), pi.duoable.end());
+        std::stable_sort(dcand.begin(), dcand.end(), [&](const Group& x, const Group& y) {
+            if ((x.key & 1) != (y.key & 1)) return (x.key & 1) < (y.key & 1);
+            const int32_t rx = rank[x.key >> 1], ry = rank[y.key >> 1];
+            return rx != ry ? rx < ry : x.key < y.key;
+        });
+        // The union of two target lists in rank order (the j-th occurrence of
+        // a target in one list meets its j-th occurrence in the other):
+        // emit(target, output index in A or -1, output index in B or -1).
+        auto duo_merge = [&](const Group& A, const Group* B, auto&& emit) -> int64_t {
+            const std::vector<Fast>& fa = parts[A.part].fast;
+            const std::vector<Fast>* fb = B ? &parts[B->part].fast : nullptr;
+            int64_t pa = A.begin, pb = B ? B->begin : 0;
+            const int64_t eb = B ? B->end : 0;
+            int64_t u = 0;
+            while (pa < A.end || pb < eb) {
+                const int32_t ra = pa < A.end ? rank[fa[pa].target] : INT32_MAX;
+                const int32_t rb = pb < eb ? rank[(*fb)[pb].target] : INT32_MAX;
+                int32_t tgt = 0, ia = -1, ib = -1;
+                if (ra <= rb) tgt = fa[pa].target, ia = fa[pa].index, ++pa;
+                if (rb <= ra) tgt = (*fb)[pb].target, ib = (*fb)[pb].index, ++pb;
+                emit(tgt, ia, ib);
+                ++u;
+            }
+            return u;
+        };
+        std::vector<int64_t> next_uni(dcand.size(), 0);   // union of candidates i and i + 1
+        const int TD = std::min<int>((int)parts.size(), std::max<int>(1, (int)dcand.size()));
+        ok = run_parallel(TD, [&](int t) {
+            for (size_t i = (size_t)t; i + 1 < dcand.size(); i += (size_t)TD)
+                next_uni[i] = duo_merge(dcand[i], &dcand[i + 1], [](int32_t, int32_t, int32_t) {});
+        });
+        if (!ok) return fail(MSA_ERR_OUT_OF_MEMORY, "host allocation failed while planning");
+        struct Duo { size_t a, b; int64_t uni, rec; };    // b == a: a query alone
+        std::vector<Duo> duos;
+        std::vector<size_t> alone;
+        int64_t paired = 0, alone_pairs = 0;
+        for (size_t i = 0; i < dcand.size();) {
+            const int64_t na = dcand[i].end - dcand[i].begin;
+            if (i + 1 < dcand.size()) {
+                const int64_t la = seqs[dcand[i].key >> 1].first_end, lb = seqs[dcand[i + 1].key >> 1].first_end;
+                const int64_t nb = dcand[i + 1].end - dcand[i + 1].begin;
+                if (20 * next_uni[i] * std::max(la, lb) <= 13 * (na * la + nb * lb)) {
+                    duos.push_back({i, i + 1, next_uni[i], 0});
+                    paired += na + nb;
+                    i += 2;
+                    continue;
+                }
+            }
+            alone.push_back(i);
+            alone_pairs += na;
+            ++i;
+        }
+        const bool alone_in_duo = !duos.empty() && 10 * alone_pairs <= paired;
+        std::vector<uint8_t> in_duo(duos.empty() && !alone_in_duo ? 0 : (size_t)2 * nseq, 0);
+        for (const size_t i : alone) {
+            if (alone_in_duo) duos.push_back({i, i, dcand[i].end - dcand[i].begin, 0});
+            else pitems[dcand[i].part].kept += dcand[i].end - dcand[i].begin;
+        }
+        for (const Duo& d : duos) {
+            in_duo[dcand[d.a].key] = 1;
+            in_duo[dcand[d.b].key] = 1;
+        }
+        constexpr int kDuoTargets = 16;    // targets per duo item: one per lane group of a workgroup
+        constexpr int64_t kDuoItemInts2 = 1 + 2 * kDuoTargets;
         // Target list: the parts' unstacked groups in part order, then per
         // stack its record, its union of targets and its output table (one
-        // row per member, in int32 units of the list).
+        // row per member, in int32 units of the list), then per duo its items
+        // (each a record and two entries per target).
         std::vector<int64_t> kept_at(parts.size() + 1, 0);
         for (size_t t = 0; t < parts.size(); ++t) kept_at[t + 1] = kept_at[t] + pitems[t].kept;
         int64_t ntargets = kept_at.back();
@@ -561,6 +643,10 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
             st.at = st.rec + 1 + 2 * (int64_t)st.m;
             st.souts = 2 * (st.at + st.uni);
             ntargets = st.at + st.uni + ((int64_t)st.m * st.uni + 1) / 2;
+        }
+        for (Duo& d : duos) {
+            d.rec = ntargets;
+            ntargets += (d.uni + kDuoTargets - 1) / kDuoTargets + 2 * d.uni;
         }
         if (2 * ntargets >= (int64_t(1) << 31))
             return fail(MSA_ERR_TOO_LARGE, "stacked target lists exceed 2^30 entries");
@@ -578,7 +664,7 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
                 const int32_t q = fast[k].query >> 1;
                 const int32_t lq = seqs[q].first_end;
                 const int cfg = group_cfg(lq);
-                if (stackable(lq, cfg)) {
+                if (stackable(lq, cfg) || (!in_duo.empty() && in_duo[fast[k].query])) {
                     k = e;
                     continue;
                 }
@@ -593,6 +679,7 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
                 for (size_t u = k; u < e; ++u) {
                     hp.targets[at++] = make_int2(fast[u].target, fast[u].index);
                     out.max_t = std::max(out.max_t, seqs[fast[u].target].first_end);
+                    if (pk) out.packed_cells += (int64_t)lq * seqs[fast[u].target].first_end;
                 }
                 for (int64_t s = first; s < at; s += item_targets) {
                     const int32_t cnt = (int32_t)std::min<int64_t>(item_targets, at - s);
@@ -645,6 +732,53 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
             pi.used |= sused[t];
             hp.info.stacked_pairs += spairs[t];
         }
+        // Duos: items (duo d's go to the part d % TD), the union entries of
+        // both queries, and the duo item records.
+        std::vector<PartItems> ditems((size_t)TD);
+        ok = run_parallel(TD, [&](int t) {
+            PartItems& out = ditems[(size_t)t];
+            for (size_t s = (size_t)t; s < duos.size(); s += (size_t)TD) {
+                const Duo& d = duos[s];
+                const Group& A = dcand[d.a];
+                const Group* B = d.b != d.a ? &dcand[d.b] : nullptr;
+                const int32_t la = seqs[A.key >> 1].first_end, lb = B ? seqs[B->key >> 1].first_end : 0;
+                const int64_t lmax = std::max(la, lb);
+                int64_t u = 0;
+                duo_merge(A, B, [&](int32_t tgt, int32_t ia, int32_t ib) {
+                    const int64_t base = d.rec + (u / kDuoTargets) * kDuoItemInts2 + 2 * (u % kDuoTargets);
+                    hp.targets[base + 1] = make_int2(tgt, ia);
+                    hp.targets[base + 2] = make_int2(tgt, ib);
+                    const int32_t lt = seqs[tgt].first_end;
+                    out.max_t = std::max(out.max_t, lt);
+                    if (ia >= 0) ++out.packed, out.packed_cells += (int64_t)la * lt;
+                    if (ib >= 0) ++out.packed, out.packed_cells += (int64_t)lb * lt;
+                    ++u;
+                });
+                const int32_t flags = (A.key & 1) | (kDuoQueue << 1);
+                const int64_t rows = (lmax + 2 * 16 - 1) / (2 * 16) * (2 * 16);
+                for (int64_t j = 0; j * kDuoTargets < d.uni; ++j) {
+                    const int64_t rec = d.rec + j * kDuoItemInts2;
+                    const int32_t cnt = (int32_t)std::min<int64_t>(kDuoTargets, d.uni - j * kDuoTargets);
+                    hp.targets[rec] = B ? make_int2(B->key >> 1, B->key & 1) : make_int2(-1, 0);
+                    const int64_t chunks = (seqs[hp.targets[rec + 1].x].first_end + 2 * 16 - 2) / 16;
+                    out.items.push_back({make_int4(A.key >> 1, (int)rec, cnt, flags), rows * chunks * 16});
+                }
+                out.used |= 1u;
+                if (lmax > 16 * 32) out.need_lines = out.packed_lines = true;
+            }
+        });
+        if (!ok) return fail(MSA_ERR_OUT_OF_MEMORY, "host allocation failed while planning");
+        for (PartItems& di : ditems) {
+            hp.info.duo_pairs += di.packed;
+            hp.info.duo_cells += di.packed_cells;
+            pitems[0].items.insert(pitems[0].items.end(), di.items.begin(), di.items.end());
+            pitems[0].max_t = std::max(pitems[0].max_t, di.max_t);
+            pitems[0].used |= di.used;
+            pitems[0].need_lines = pitems[0].need_lines || di.need_lines;
+            pitems[0].packed += di.packed;
+            pitems[0].packed_cells += di.packed_cells;
+            pitems[0].packed_lines = pitems[0].packed_lines || di.packed_lines;
+        }
         for (Part& P : parts) std::vector<Fast>().swap(P.fast);
         std::vector<Item> items;
         int32_t max_t = 1;
@@ -656,6 +790,7 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
             need_lines = need_lines || pi.need_lines;
             used |= pi.used;
             hp.info.packed_pairs += pi.packed;
+            hp.info.packed_cells += pi.packed_cells;
             hp.packed_lines = hp.packed_lines || pi.packed_lines;
         }
         // One queue per tile configuration and the packed queue (each has its
@@ -663,7 +798,7 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
         // planning threads sort contiguous ranges, then adjacent ranges are
         // merged (stably) level by level.
         auto before = [](const Item& x, const Item& y) {
-            const int cx = (x.v.w >> 1) & 3, cy = (y.v.w >> 1) & 3;
+            const int cx = (x.v.w >> 1) & kQueueMask, cy = (y.v.w >> 1) & kQueueMask;
             return cx != cy ? cx < cy : x.cost > y.cost;
         };
         const int T = (int)parts.size();
@@ -687,7 +822,7 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
         hp.items.resize(items.size());
         for (size_t u = 0; u < items.size(); ++u) {
             hp.items[u] = items[u].v;
-            ++hp.batch_first[((items[u].v.w >> 1) & 3) + 1];
+            ++hp.batch_first[((items[u].v.w >> 1) & kQueueMask) + 1];
         }
         for (int c = 0; c < kBatchQueues; ++c) {
             const int64_t n = hp.batch_first[c + 1];
@@ -908,7 +1043,7 @@ int launch_plan(const HostPlan& hp, const DataLayout& lay, const void* d_data, c
         if (n == 0) continue;
         MSA_TRY(batched_launch(c, hp.batch_grid[c], d_blocks, d_info,
                                reinterpret_cast<const int4*>(base + lay.items) + first, (int32_t)n,
-                               reinterpret_cast<const int2*>(base + lay.targets), scr.batch_queue + 16 * c, d_out,
+                               reinterpret_cast<const int2*>(base + lay.targets), scr.batch_queue + 8 * c, d_out,
                                scr.batch_lines, std::max(hp.batch_stride, 1), hp.sxx, hp.penalty, stream));
     }
     if (!hp.long_pairs.empty()) {

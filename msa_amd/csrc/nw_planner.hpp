@@ -22,35 +22,43 @@ int seq_info_launch(const uint16_t* d_blocks, const int64_t* d_offset, const int
                     SeqInfo* d_info, hipStream_t stream);
 // Tile shapes of the batched kernel, chosen per item: `lanes` lanes score one
 // pair, each lane owning up to `rows` rows of a strip (strips of lanes*rows
-// query rows).  The index into this table travels in bits 1..2 of an item's
-// flags.
+// query rows).  The index into this table travels in bits 1..3 of an item's
+// flags (with the two packed queues below).
 struct TileConfig {
     int lanes;
     int rows;
 };
 constexpr int kTileConfigCount = 3;
 constexpr TileConfig kTileConfig[kTileConfigCount] = {{16, 32}, {16, 16}, {32, 16}};
-// Batched queues: one per tile configuration, and one more for queries of the
-// 16 x 32 shape whose cells fit the packed kernel's 16-bit cells (those that
-// fill one strip on their own and are not stacked, when max(sxx) x rows <=
-// 0x7BFF; longer ones when max(sxx) x kPackedSpan <= 0x7BFF): that kernel
-// scores two targets per lane group.  An item's queue travels in bits 1..2 of
-// its flags.
+// Batched queues: one per tile configuration, and two more for queries of the
+// 16 x 32 shape whose cells fit 16-bit packed cells (those that fill one
+// strip on their own and are not stacked, when max(sxx) x rows <= 0x7BFF;
+// longer ones when max(sxx) x kPackedSpan <= 0x7BFF).  The packed kernel
+// scores one query against two targets per lane group; the duo kernel two
+// queries against one target per lane group, its workgroups taking whole
+// items.  An item's queue travels in bits 1..3 of its flags.
 constexpr int kPackedQueue = kTileConfigCount;
-constexpr int kBatchQueues = kTileConfigCount + 1;
+constexpr int kDuoQueue = kTileConfigCount + 1;
+constexpr int kBatchQueues = kTileConfigCount + 2;
+constexpr int kQueueMask = 7;                  // (flags >> 1) & kQueueMask: the item's queue
+// Duo items: item.x is the first query, item.y the index in the target list
+// of the item's record {second query or -1, its transpose flag}, followed by
+// {target, first query's output index} and {target, second query's output
+// index} for each of the item.z targets (-1: that query does not pair with
+// the target).
 constexpr int32_t kPackedMaxScore = 0x7BFF;   // largest packed cell value
 constexpr int kPackedMinRows = 225;            // packed queries: first-strip heights of 16..32 rows per lane
 constexpr int kPackedSpan = 592;               // a rebased strip's cells span at most kPackedSpan x max(sxx)
 constexpr int kMaxTileBytes = 512;     // largest lanes*rows: one code's profile block
 constexpr int kRowStep = 2;            // rows-per-lane granularity of a query's first strip
-// Stacked items (bit 3 of an item's flags): item.x is the index, in the
+// Stacked items (bit 4 of an item's flags): item.x is the index, in the
 // target list, of the stack's record: {members m, rows of the stack (queries
 // plus separators)}, then per member {query, K} and {offset of its row of
 // output indices (in int32 units of the target list), rows * 2 + transpose};
 // the stack's targets are {sequence, index in the stack's target union}, and
 // its output table (per member, an output index per union target, -1 where
 // the member does not pair with it) sits in the target list too.
-constexpr int kStackedFlag = 8;
+constexpr int kStackedFlag = 16;
 constexpr int kMaxStackMembers = 16;
 // A batched lane group's strip-boundary line: kLineGuard ints in front of
 // column 0 and room for every column a strip's wavefront reaches (up to 2G

@@ -680,12 +680,14 @@ def test_long_pair_beyond_the_line_limit_is_refused():
     assert "boundary line" in str(e.value)
 
 
+@pytest.mark.parametrize("duo", [True, False])
 @pytest.mark.parametrize("table", ["blosum62", "pam250", "default", "asymmetric"])
-def test_packed_cells_match_int32_and_oracle(table):
+def test_packed_cells_match_int32_and_oracle(table, duo):
     """Queries that fill one 16 x 32 strip on their own (225..512 rows) are
-    scored two targets per lane group in packed 16-bit cells: every score
-    equals the oracle's and the all-int32 plan's, with odd target counts
-    (a lane group with one target) and targets of every length."""
+    scored in packed 16-bit cells, two queries against one target per lane
+    group (duo) or one query against two targets: every score equals the
+    oracle's and the all-int32 plan's, with odd target counts (a lane group
+    with one target) and targets of every length."""
     rng = np.random.default_rng(71 + len(table))
     if table == "asymmetric":
         T, p = H.tables()["blosum62"]
@@ -695,15 +697,18 @@ def test_packed_cells_match_int32_and_oracle(table):
         T, p = H.tables()[table]
     lengths = list(rng.integers(225, 513, 30)) + list(rng.integers(1, 1500, 13))
     db = W.pack([rng.integers(0, 24, int(n)).astype(np.uint8) for n in lengths])
-    got = assert_oracle(db, T, p, packed=True)
+    got = assert_oracle(db, T, p, packed=True, duo=duo)
     assert np.array_equal(got, gpu_scores(db, T, p, packed=False))
-    assert plan_info(db, T, p, packed=True).packed_pairs > 0
+    info = plan_info(db, T, p, packed=True, duo=duo)
+    assert info.packed_pairs > 0
+    assert (info.duo_pairs > 0) == duo
     assert plan_info(db, T, p, packed=False).packed_pairs == 0
     # a plan this small keeps its single launch unless packing is asked for
     assert plan_info(db, T, p).packed_pairs == 0
 
 
-def test_packed_cells_at_the_top_of_their_range():
+@pytest.mark.parametrize("duo", [True, False])
+def test_packed_cells_at_the_top_of_their_range(duo):
     """A packed cell holds at most 0x7BFF: with max(sxx) = 61 a 512-row query
     is packed (61 x 512 = 31,232), and a pair of two identical all-W
     sequences reaches that bound; with max(sxx) = 62 (62 x 512 = 31,744) the
@@ -716,9 +721,9 @@ def test_packed_cells_at_the_top_of_their_range():
     seqs += [rng.integers(0, 24, int(n)).astype(np.uint8) for n in rng.integers(225, 513, 6)]
     db = W.pack(seqs)
     for p, packed in ((25, True), (26, False)):
-        got = assert_oracle(db, T, p, packed=True)
+        got = assert_oracle(db, T, p, packed=True, duo=duo)
         assert int(got[0]) == 512 * 11             # pair (1, 0): the identical all-W pair
-        info = plan_info(db, T, p, packed=True)
+        info = plan_info(db, T, p, packed=True, duo=duo)
         if packed:
             assert info.packed_pairs > 0
         else:
@@ -738,8 +743,11 @@ def test_packed_cells_in_the_uneven_set():
     flat = N.Plan(w.blocks, w.offsets, w.sizes, w.table, w.penalty, opts=N.options(N.KERNEL_AUTO, packed=False))
     out = N.DeviceBuffer(4 * w.npairs)
     try:
-        assert packed.info().packed_pairs > 100000
-        assert flat.info().packed_pairs == 0
+        info = packed.info()
+        assert info.packed_pairs > 100000
+        assert info.duo_pairs > 0.9 * info.packed_pairs   # nearly every packed query finds a partner
+        assert 225 * 50 * info.packed_pairs <= info.packed_cells < info.cells   # queries >= 225, targets >= 50
+        assert flat.info().packed_pairs == 0 and flat.info().packed_cells == 0
         flat.execute(out.ptr)
         exp = out.download(np.int32, w.npairs)
         for _ in range(2):
@@ -752,7 +760,8 @@ def test_packed_cells_in_the_uneven_set():
         flat.close()
 
 
-def test_packed_cells_longer_than_a_strip_at_the_top_of_their_range():
+@pytest.mark.parametrize("duo", [True, False])
+def test_packed_cells_longer_than_a_strip_at_the_top_of_their_range(duo):
     """Queries longer than one strip take packed cells when their rebased
     strips' cells fit: max(sxx) x 592 <= 0x7BFF.  All-W sequences of 2000 and
     3000 residues with max(sxx) = 53 (blosum62, gap 21) make every cell grow
@@ -766,8 +775,64 @@ def test_packed_cells_longer_than_a_strip_at_the_top_of_their_range():
     seqs += [rng.integers(0, 24, int(n)).astype(np.uint8) for n in (700, 1300, 2600)]
     db = W.pack(seqs)
     for p, packed in ((21, True), (22, False)):
-        got = assert_oracle(db, T, p, packed=True)
+        got = assert_oracle(db, T, p, packed=True, duo=duo)
         assert int(got[2]) == 3000 * 11           # pair (2, 1): identical all-W 3000-residue sequences
         assert np.array_equal(got, gpu_scores(db, T, p, packed=False))
-        info = plan_info(db, T, p, packed=True)
+        info = plan_info(db, T, p, packed=True, duo=duo)
         assert (info.packed_pairs > 0) == packed
+        if packed:                                # every query of the three all-W pairs is longer than 2000
+            assert info.packed_cells >= 2000 * 3000 * 2 + 3000 * 3000
+
+
+@pytest.mark.parametrize("table", ["blosum62", "asymmetric"])
+def test_duo_items_with_explicit_pair_lists(table):
+    """Duo items from an explicit pair list: queries whose target lists only
+    partly agree (entries -1 for the half that does not pair), a pair named
+    twice, both orientations of an asymmetric table in one duo, queries of
+    different strip counts sharing items (the shorter padded with neutral
+    rows on top), and queries left alone (high halves idle).  Every score
+    equals the oracle's and the int32 plan's."""
+    rng = np.random.default_rng(23)
+    T, p = H.tables()["blosum62"]
+    if table == "asymmetric":
+        T = T.copy()
+        T[0, 1], T[1, 0], T[6, 13], T[19, 4] = 7, -3, 2, -4
+    lengths = [300, 310, 505, 530, 1030, 1100, 2100, 2150, 700, 260, 990, 1500]
+    lengths += list(rng.integers(600, 2400, 28))
+    db = W.pack([rng.integers(0, 24, int(n)).astype(np.uint8) for n in lengths])
+    n = len(lengths)
+    pairs = [(i, j) for i in range(n) for j in range(i) if rng.random() < 0.8]
+    pairs += [(5, 4), (4, 5), (7, 6), (3, 2), (3, 2)]
+    pairs = np.array(pairs, dtype=np.int32)
+    got = assert_oracle(db, T, p, pairs=pairs, packed=True)
+    assert np.array_equal(got, gpu_scores(db, T, p, pairs=pairs, packed=False))
+    assert np.array_equal(got, gpu_scores(db, T, p, pairs=pairs, packed=True, duo=False))
+    info = plan_info(db, T, p, pairs=pairs, packed=True)
+    assert info.duo_pairs > 0
+
+
+def test_duo_items_in_graph_replays_after_other_plans():
+    """A duo plan replayed as a graph after plans of the other kernels ran
+    on the same stream scores the same every time (its lines and queue
+    counters start clean)."""
+    T, p = H.tables()["blosum62"]
+    rng = np.random.default_rng(31)
+    lengths = list(rng.integers(230, 1800, 60))
+    db = W.pack([rng.integers(0, 24, int(n)).astype(np.uint8) for n in lengths])
+    exp = H.oracle_scores(db, T, p)
+    duo = N.Plan(*db, T, p, opts=N.options(N.KERNEL_AUTO, use_graph=True, packed=True))
+    other = N.Plan(*db, T, p, opts=N.options(N.KERNEL_AUTO, packed=True, duo=False))
+    out = N.DeviceBuffer(4 * len(exp))
+    try:
+        assert duo.info().duo_pairs > 0
+        for _ in range(3):
+            out.upload(np.full(len(exp), -7, dtype=np.int32))
+            other.execute(out.ptr)
+            assert np.array_equal(out.download(np.int32, len(exp)).astype(np.float32), exp)
+            out.upload(np.full(len(exp), -7, dtype=np.int32))
+            duo.execute(out.ptr)
+            assert np.array_equal(out.download(np.int32, len(exp)).astype(np.float32), exp)
+    finally:
+        out.close()
+        duo.close()
+        other.close()
