@@ -98,6 +98,9 @@ struct BatchArgs {
                                 // stack's union}, and the stack records and output tables
     int32_t nitems;
     int32_t* queue;
+    const int4* items2;         // duo kernel: the 16 x 32 queue, scored after the duo items
+    int32_t nitems2;
+    int32_t* queue2;
     int32_t* out;
     int32_t* lines;             // one line of line_stride ints per lane group
     int32_t line_stride;
@@ -1250,6 +1253,16 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(3))
             __syncthreads();
             if (it >= a.nitems) break;
             run_item_duo(a, a.items[it], lds_base, tab, rings, lines, wave, lane);
+        }
+        // Then the 16 x 32 items, one per wave, each wave in its own profile
+        // region and its first ring bytes: the duo items of other workgroups
+        // may still run, and these fill their tail.
+        while (a.nitems2 > 0) {
+            int it = 0;
+            if (lane == 0) it = atomicAdd(a.queue2, 1);
+            it = __shfl(it, 0);
+            if (it >= a.nitems2) break;
+            run_item<kTileConfig[0].lanes, kTileConfig[0].rows>(a, a.items2[it], prof, tab, rings, lines, lane);
         }
     } else {
         while (true) {

@@ -454,7 +454,9 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
         // rows].
         // Packed items run in a launch of their own: a plan too small to give
         // every resident wave four items of 8 targets keeps its single launch.
-        const bool packing = opt.packed_cells > 0 || (opt.packed_cells == 0 && item_targets >= 8);
+        // With duo items (below) the test waits for the pair count.
+        bool packing = opt.packed_cells > 0 ||
+                       (opt.packed_cells == 0 && (opt.duo_queries >= 0 || item_targets >= 8));
         // A query longer than one strip is packed when its rebased strips'
         // cells fit (at most kPackedSpan x max(sxx) above their base).
         auto packable = [&](int32_t lq, int cfg) {
@@ -495,6 +497,22 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
             }
         });
         if (!ok) return fail(MSA_ERR_OUT_OF_MEMORY, "host allocation failed while planning");
+        // Duo items (32 pairs each) run a workgroup each: a plan with fewer
+        // than 1.5 of them per resident workgroup scores its packable
+        // queries in int32 cells unless packing is asked for (its few, long
+        // duo items would set the time).
+        if (duo && opt.packed_cells == 0) {
+            int64_t cand_pairs = 0;
+            for (const PartItems& pi : pitems)
+                for (const Group& g : pi.duoable) cand_pairs += g.end - g.begin;
+            if (cand_pairs < (int64_t)48 * per_cu * dev.compute_units) {
+                packing = false;
+                for (PartItems& pi : pitems) {
+                    for (const Group& g : pi.duoable) pi.kept += g.end - g.begin;
+                    pi.duoable.clear();
+                }
+            }
+        }
         // Stacks: the stackable groups of each tile shape in rank order (length
         // descending), packed greedily while the strip holds them.
         std::vector<Group> cand;
@@ -764,7 +782,10 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
                     out.items.push_back({make_int4(A.key >> 1, (int)rec, cnt, flags), rows * chunks * 16});
                 }
                 out.used |= 1u;
-                if (lmax > 16 * 32) out.need_lines = out.packed_lines = true;
+                // The duo kernel's waves own two lines per lane group (also
+                // for the 16 x 32 items it takes on).
+                out.packed_lines = true;
+                if (lmax > 16 * 32) out.need_lines = true;
             }
         });
         if (!ok) return fail(MSA_ERR_OUT_OF_MEMORY, "host allocation failed while planning");
@@ -945,6 +966,7 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
     }
     int batched_launches = 0;
     for (int c = 0; c < kBatchQueues; ++c) batched_launches += hp.batch_grid[c] > 0 ? 2 : 0;
+    if (hp.batch_grid[0] > 0 && hp.batch_grid[kDuoQueue] > 0) batched_launches -= 2;   // queue 0 rides in the duo launch
     hp.info.launches = (int32_t)(batched_launches + (hp.long_pairs.empty() ? 0 : 2) +
                                  (hp.plain_index.empty() ? 0 : 1));
     return MSA_OK;
@@ -1038,13 +1060,19 @@ int launch_plan(const HostPlan& hp, const DataLayout& lay, const void* d_data, c
     const char* base = static_cast<const char*>(d_data);
     const uint16_t* d_blocks = d_store ? d_store : reinterpret_cast<const uint16_t*>(base + lay.blocks);
     const SeqInfo* d_info = reinterpret_cast<const SeqInfo*>(base + lay.seqinfo);
+    // With duo items, the 16 x 32 queue rides in the duo launch.
+    const bool merged = hp.batch_first[kDuoQueue + 1] > hp.batch_first[kDuoQueue] && hp.batch_first[1] > hp.batch_first[0];
+    const int4* d_items = reinterpret_cast<const int4*>(base + lay.items);
     for (int c = 0; c < kBatchQueues; ++c) {
         const int64_t first = hp.batch_first[c], n = hp.batch_first[c + 1] - first;
-        if (n == 0) continue;
-        MSA_TRY(batched_launch(c, hp.batch_grid[c], d_blocks, d_info,
-                               reinterpret_cast<const int4*>(base + lay.items) + first, (int32_t)n,
-                               reinterpret_cast<const int2*>(base + lay.targets), scr.batch_queue + 8 * c, d_out,
-                               scr.batch_lines, std::max(hp.batch_stride, 1), hp.sxx, hp.penalty, stream));
+        if (n == 0 || (merged && c == 0)) continue;
+        const bool with0 = merged && c == kDuoQueue;
+        MSA_TRY(batched_launch(c, with0 ? std::max(hp.batch_grid[c], hp.batch_grid[0]) : hp.batch_grid[c], d_blocks,
+                               d_info, d_items + first, (int32_t)n, reinterpret_cast<const int2*>(base + lay.targets),
+                               scr.batch_queue + 8 * c, d_out, scr.batch_lines, std::max(hp.batch_stride, 1), hp.sxx,
+                               hp.penalty, stream, with0 ? d_items + hp.batch_first[0] : nullptr,
+                               with0 ? (int32_t)(hp.batch_first[1] - hp.batch_first[0]) : 0,
+                               with0 ? scr.batch_queue : nullptr));
     }
     if (!hp.long_pairs.empty()) {
         LongLaunch l;
