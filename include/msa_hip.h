@@ -146,13 +146,14 @@ typedef struct {
                                 table (16 lanes x 32 rows for every query, long from 8192) */
     int single_query_strips; /* 1: every batched strip holds one query; 0 (default): queries
                                 that fill at most half a strip are stacked, several to a strip */
-    int packed_cells;        /* queries of at least 225 residues that are not stacked score two
-                                targets per lane group in packed 16-bit cells when the cells stay
-                                within 15 bits (a query of one strip: every score of its pairs; a
-                                longer query: the growth of one 592-column stretch, the cells being
-                                rebased as they go): 0 (default) in plans large enough to give
-                                every resident wave four items of 8 targets (a small plan keeps its
-                                single launch); 1 in every plan; -1 never (every cell an int32) */
+    int packed_cells;        /* queries of at least 225 residues that are not stacked are scored
+                                in packed 16-bit cells when the cells stay within 15 bits (a
+                                query of one strip: every score of its pairs; a longer query: the
+                                growth of one 592-column stretch, the cells being rebased as they
+                                go): 0 (default) in large enough plans (with duo_queries: at least
+                                1.5 duo items of 32 pairs per resident workgroup; without: four
+                                items of 8 targets per resident wave); 1 in every plan; -1 never
+                                (every cell an int32) */
     int duo_queries;         /* packed queries are scored two at a time against each target (one
                                 query per half of a packed cell) where their target lists agree:
                                 0 (default) yes, -1 no (two targets per lane group instead) */

@@ -467,8 +467,14 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
         };
         // Packed queries go in pairs to the duo kernel (two queries against
         // one target per lane group, about 1.1 VALU operations per cell
-        // against 1.6 in the packed kernel) when their target lists agree.
+        // against 1.6 in the packed kernel) when their target lists agree,
+        // from kDuoMinRows rows on (also queries that would stack).
         const bool duo = packing && opt.duo_queries >= 0;
+        auto duo_ok = [&](int32_t lq, int cfg) {
+            if (!packing || cfg != 0 || lq < kDuoMinRows) return false;
+            if (lq <= 16 * 32) return (int64_t)max_sxx * lq <= kPackedMaxScore;
+            return (int64_t)max_sxx * kPackedSpan <= kPackedMaxScore;
+        };
         auto item_cost = [&](int64_t rows_in, const TileConfig& tc, const int2* tg, int32_t cnt) {
             const int groups = 64 / tc.lanes;
             const int64_t step = (int64_t)kRowStep * tc.lanes;
@@ -490,8 +496,8 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
                 while (e < fast.size() && fast[e].query == fast[k].query) ++e;
                 const int32_t lq = seqs[fast[k].query >> 1].first_end;
                 const int cfg = group_cfg(lq);
-                if (stackable(lq, cfg)) out.stackable.push_back({t, fast[k].query, cfg, (int64_t)k, (int64_t)e});
-                else if (duo && packable(lq, cfg)) out.duoable.push_back({t, fast[k].query, cfg, (int64_t)k, (int64_t)e});
+                if (duo && duo_ok(lq, cfg)) out.duoable.push_back({t, fast[k].query, cfg, (int64_t)k, (int64_t)e});
+                else if (stackable(lq, cfg)) out.stackable.push_back({t, fast[k].query, cfg, (int64_t)k, (int64_t)e});
                 else out.kept += (int64_t)(e - k);
                 k = e;
             }
@@ -508,7 +514,10 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
             if (cand_pairs < (int64_t)48 * per_cu * dev.compute_units) {
                 packing = false;
                 for (PartItems& pi : pitems) {
-                    for (const Group& g : pi.duoable) pi.kept += g.end - g.begin;
+                    for (const Group& g : pi.duoable) {
+                        if (stackable(seqs[g.key >> 1].first_end, g.cfg)) pi.stackable.push_back(g);
+                        else pi.kept += g.end - g.begin;
+                    }
                     pi.duoable.clear();
                 }
             }
@@ -638,9 +647,11 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
             ++i;
         }
         const bool alone_in_duo = !duos.empty() && 10 * alone_pairs <= paired;
-        std::vector<uint8_t> in_duo(duos.empty() && !alone_in_duo ? 0 : (size_t)2 * nseq, 0);
+        std::vector<uint8_t> in_duo(dcand.empty() ? 0 : (size_t)2 * nseq, 0);
+        // (A short query alone always takes a duo item: the stacks are made.)
         for (const size_t i : alone) {
-            if (alone_in_duo) duos.push_back({i, i, dcand[i].end - dcand[i].begin, 0});
+            if (alone_in_duo || stackable(seqs[dcand[i].key >> 1].first_end, dcand[i].cfg))
+                duos.push_back({i, i, dcand[i].end - dcand[i].begin, 0});
             else pitems[dcand[i].part].kept += dcand[i].end - dcand[i].begin;
         }
         for (const Duo& d : duos) {

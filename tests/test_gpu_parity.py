@@ -643,11 +643,14 @@ def test_stacked_strips_explicit_list_with_repeats():
 
 
 def test_stacked_strips_in_the_uneven_set_graph_replays():
-    """uneven-2000 with stacking: more than half of its pairs are scored in
-    stacked strips; graph replays equal the unstacked plan on every pair."""
+    """uneven-2000 with stacking and int32 cells: more than half of its pairs
+    are scored in stacked strips; graph replays equal the unstacked plan on
+    every pair."""
     w = W.uneven_batch(H.tables())
-    stacked = N.Plan(w.blocks, w.offsets, w.sizes, w.table, w.penalty, opts=N.options(N.KERNEL_AUTO, use_graph=True))
-    flat = N.Plan(w.blocks, w.offsets, w.sizes, w.table, w.penalty, opts=N.options(N.KERNEL_AUTO, stack=False))
+    stacked = N.Plan(w.blocks, w.offsets, w.sizes, w.table, w.penalty,
+                     opts=N.options(N.KERNEL_AUTO, use_graph=True, packed=False))
+    flat = N.Plan(w.blocks, w.offsets, w.sizes, w.table, w.penalty,
+                  opts=N.options(N.KERNEL_AUTO, stack=False, packed=False))
     out = N.DeviceBuffer(4 * w.npairs)
     try:
         assert stacked.info().stacked_pairs > w.npairs // 2
