@@ -98,15 +98,15 @@ struct BatchArgs {
                                 // stack's union}, and the stack records and output tables
     int32_t nitems;
     int32_t* queue;
-    const int4* items2;         // duo kernel: the 16 x 32 queue, scored after the duo items
-    int32_t nitems2;
-    int32_t* queue2;
     int32_t* out;
     int32_t* lines;             // one line of line_stride ints per lane group
     int32_t line_stride;
     int32_t penalty;
     int32_t margin;             // packed kernel: 16 x the largest sxx (how far below the line a base is set)
     Sxx sxx;
+    const int4* items2;         // duo kernel: the 16 x 32 queue, scored after the duo items
+    int32_t* queue2;
+    int32_t nitems2;
 };
 
 // Calls f(integral_constant<int, U>) for U = B..E-1, so that U can name DPP
