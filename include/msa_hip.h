@@ -151,7 +151,7 @@ typedef struct {
                                 query of one strip: every score of its pairs; a longer query: the
                                 growth of one 592-column stretch, the cells being rebased as they
                                 go): 0 (default) in large enough plans (with duo_queries: at least
-                                1.5 duo items of 32 pairs per resident workgroup; without: four
+                                2 duo items of 32 pairs per resident workgroup; without: four
                                 items of 8 targets per resident wave); 1 in every plan; -1 never
                                 (every cell an int32) */
     int duo_queries;         /* packed queries are scored two at a time against each target (one

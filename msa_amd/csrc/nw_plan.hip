@@ -504,14 +504,16 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
         });
         if (!ok) return fail(MSA_ERR_OUT_OF_MEMORY, "host allocation failed while planning");
         // Duo items (32 pairs each) run a workgroup each: a plan with fewer
-        // than 1.5 of them per resident workgroup scores its packable
-        // queries in int32 cells unless packing is asked for (its few, long
-        // duo items would set the time).
+        // than 2 of them per resident workgroup scores its packable queries
+        // in int32 cells unless packing is asked for (its few, long duo
+        // items would set the time: uneven sets of 400 sequences, 1.6 duo
+        // items per workgroup, ran 15% slower packed, of 480, 2.2 per
+        // workgroup, 34% faster).
         if (duo && opt.packed_cells == 0) {
             int64_t cand_pairs = 0;
             for (const PartItems& pi : pitems)
                 for (const Group& g : pi.duoable) cand_pairs += g.end - g.begin;
-            if (cand_pairs < (int64_t)48 * per_cu * dev.compute_units) {
+            if (cand_pairs < (int64_t)64 * per_cu * dev.compute_units) {
                 packing = false;
                 for (PartItems& pi : pitems) {
                     for (const Group& g : pi.duoable) {
