@@ -839,3 +839,32 @@ def test_duo_items_in_graph_replays_after_other_plans():
         out.close()
         duo.close()
         other.close()
+
+
+def test_duo_launch_with_multi_strip_int32_items():
+    """With max(sxx) = 55 (blosum62, gap 22) one-strip queries fit 16-bit
+    cells and take duo items, while queries longer than a strip do not
+    (55 x 592 > 0x7BFF) and take int32 items: the duo launch scores those
+    after its duo items, each wave in its own part of the workgroup's LDS and
+    with its own strip-boundary lines.  Every score equals the oracle's and
+    the all-int32 plan's, over graph replays too."""
+    T, _ = H.tables()["blosum62"]
+    p = 22
+    rng = np.random.default_rng(41)
+    lengths = list(rng.integers(230, 512, 36)) + list(rng.integers(600, 1700, 12))
+    db = W.pack([rng.integers(0, 24, int(n)).astype(np.uint8) for n in lengths])
+    info = plan_info(db, T, p, packed=True)
+    assert info.duo_pairs > 0
+    assert info.batched_pairs > info.packed_pairs + info.stacked_pairs   # int32 items beside the duo items
+    got = assert_oracle(db, T, p, packed=True)
+    assert np.array_equal(got, gpu_scores(db, T, p, packed=False))
+    plan = N.Plan(*db, T, p, opts=N.options(N.KERNEL_AUTO, use_graph=True, packed=True))
+    out = N.DeviceBuffer(4 * len(got))
+    try:
+        for _ in range(3):
+            out.upload(np.full(len(got), -7, dtype=np.int32))
+            plan.execute(out.ptr)
+            assert np.array_equal(out.download(np.int32, len(got)), got)
+    finally:
+        out.close()
+        plan.close()
