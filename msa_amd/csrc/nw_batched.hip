@@ -15,10 +15,11 @@ int launch_16x32(int grid, const BatchArgs& a, hipStream_t stream) { return kern
 
 namespace {
 
-__global__ void queue_reset_kernel(int32_t* queue, int32_t* queue2)
+__global__ void queue_reset_kernel(int32_t* queue, int32_t* queue2, int32_t* queue3)
 {
     if (threadIdx.x == 0) queue[0] = 0;
     if (threadIdx.x == 1 && queue2 != nullptr) queue2[0] = 0;
+    if (threadIdx.x == 2 && queue3 != nullptr) queue3[0] = 0;
 }
 
 }  // namespace
@@ -46,13 +47,16 @@ int batched_occupancy(int* blocks_per_cu)
 int batched_launch(int config, int grid, const uint16_t* d_blocks, const SeqInfo* d_info, const int4* d_items,
                    int32_t nitems, const int2* d_targets, int32_t* d_queue, int32_t* d_out, int32_t* d_lines,
                    int32_t line_stride, const Sxx& sxx, int32_t penalty, hipStream_t stream,
-                   const int4* d_items2, int32_t nitems2, int32_t* d_queue2)
+                   const int4* d_items2, int32_t nitems2, int32_t* d_queue2, const int4* d_items3, int32_t nitems3,
+                   int32_t* d_queue3)
 {
     if (config < 0 || config >= kBatchQueues) return fail(MSA_ERR_INVALID_ARGUMENT, "no batched queue %d", config);
-    if (nitems2 > 0 && (config != kDuoQueue || d_items2 == nullptr || d_queue2 == nullptr))
-        return fail(MSA_ERR_INVALID_ARGUMENT, "only the duo launch takes a second queue");
+    if ((nitems2 > 0 && (config != kDuoQueue || d_items2 == nullptr || d_queue2 == nullptr)) ||
+        (nitems3 > 0 && (config != kDuoQueue || d_items3 == nullptr || d_queue3 == nullptr)))
+        return fail(MSA_ERR_INVALID_ARGUMENT, "only the duo launch takes more queues");
     if (nitems2 <= 0) d_queue2 = nullptr;
-    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, stream, d_queue, d_queue2);
+    if (nitems3 <= 0) d_queue3 = nullptr;
+    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, stream, d_queue, d_queue2, d_queue3);
     MSA_HIP(hipGetLastError());
     batched::BatchArgs a;
     a.blocks = d_blocks;
@@ -64,6 +68,9 @@ int batched_launch(int config, int grid, const uint16_t* d_blocks, const SeqInfo
     a.items2 = nitems2 > 0 ? d_items2 : nullptr;
     a.nitems2 = std::max(nitems2, 0);
     a.queue2 = d_queue2;
+    a.items3 = nitems3 > 0 ? d_items3 : nullptr;
+    a.nitems3 = std::max(nitems3, 0);
+    a.queue3 = d_queue3;
     a.out = d_out;
     a.lines = d_lines;
     a.line_stride = line_stride;

@@ -107,6 +107,9 @@ struct BatchArgs {
     const int4* items2;         // duo kernel: the 16 x 32 queue, scored after the duo items
     int32_t* queue2;
     int32_t nitems2;
+    int32_t nitems3;
+    const int4* items3;         // duo kernel: the packed queue, scored before the 16 x 32 queue
+    int32_t* queue3;
 };
 
 // Calls f(integral_constant<int, U>) for U = B..E-1, so that U can name DPP
@@ -704,21 +707,27 @@ __device__ __forceinline__ uint32_t pk_sub(uint32_t a, uint32_t b)
 // over 4 chunks), which the planner keeps within 0x7BFF.  Each target has its
 // own code ring of 3G bytes: virtual column i at byte i mod 2G, and columns
 // with i mod 2G < G again at i mod 2G + 2G, so that a lane reads the code for
-// step u of a chunk at a per-chunk base plus u, as in run_strip.  Returns both
-// targets' last-row cells (last lane) and their bases.
+// step u of a chunk at a per-chunk base plus u, as in run_strip.  A stacked
+// strip starts its rows at the lane's K values (`lr`, both halves alike: the
+// two targets share the stack).  Returns both targets' cells of row lr.sel
+// (row R-1 when lr.sel < 0) and their bases.
 template <int G, int R, bool LOAD, bool STORE>
 __device__ __forceinline__ uint32_t run_strip_packed(const dev::lds_u32* prof, const TargetState& ta,
                                                      const TargetState& tb, volatile dev::lds_u8* ra,
                                                      volatile dev::lds_u8* rb, const PackedLines& pl, int wchunks,
-                                                     int k, int margin, int& base_a, int& base_b)
+                                                     int k, int margin, int& base_a, int& base_b, const LaneRows& lr)
 {
     static_assert(G == 16, "packed strips use 16-lane groups");
     constexpr int W = dev::prof_words<R>;
     constexpr int CODE_WORDS = code_words<G, R>;
     constexpr int RING = 2 * G;
     uint32_t h[R];
+    {
+        const uint32_t klo = (uint32_t)lr.k_lo | ((uint32_t)lr.k_lo << 16);
+        const uint32_t khi = (uint32_t)lr.k_hi | ((uint32_t)lr.k_hi << 16);
 #pragma unroll
-    for (int r = 0; r < R; ++r) h[r] = 0;
+        for (int r = 0; r < R; ++r) h[r] = r < lr.kb ? klo : khi;
+    }
     uint32_t dtop = 0;
     base_a = 0;
     base_b = 0;
@@ -816,22 +825,31 @@ __device__ __forceinline__ uint32_t run_strip_packed(const dev::lds_u32* prof, c
         ka.advance();
         kb.advance();
     }
-    return h[R - 1];
+    // As in run_strip: the row select stays after the loop.
+    int sel = lr.sel;
+    asm volatile("" : "+v"(sel));
+    uint32_t result = h[R - 1];
+#pragma unroll
+    for (int r = 0; r < R - 1; ++r) result = r == sel ? h[r] : result;
+    return result;
 }
 
 template <bool STORE, int R = kRowStep>
 __device__ __forceinline__ uint32_t run_first_strip_packed(int r0, const dev::lds_u32* prof, const TargetState& ta,
                                                            const TargetState& tb, volatile dev::lds_u8* ra,
                                                            volatile dev::lds_u8* rb, const PackedLines& pl,
-                                                           int wchunks, int k, int margin, int& base_a, int& base_b)
+                                                           int wchunks, int k, int margin, int& base_a, int& base_b,
+                                                           const LaneRows& lr)
 {
     if constexpr (R >= 32) {
-        return run_strip_packed<16, 32, false, STORE>(prof, ta, tb, ra, rb, pl, wchunks, k, margin, base_a, base_b);
+        return run_strip_packed<16, 32, false, STORE>(prof, ta, tb, ra, rb, pl, wchunks, k, margin, base_a, base_b,
+                                                      lr);
     } else {
         if (r0 == R)
-            return run_strip_packed<16, R, false, STORE>(prof, ta, tb, ra, rb, pl, wchunks, k, margin, base_a, base_b);
+            return run_strip_packed<16, R, false, STORE>(prof, ta, tb, ra, rb, pl, wchunks, k, margin, base_a,
+                                                         base_b, lr);
         return run_first_strip_packed<STORE, R + kRowStep>(r0, prof, ta, tb, ra, rb, pl, wchunks, k, margin, base_a,
-                                                           base_b);
+                                                           base_b, lr);
     }
 }
 
@@ -860,8 +878,10 @@ __device__ __forceinline__ void run_item_packed(const BatchArgs& a, const int4 i
     const int k = lane % G;
     volatile dev::lds_u8* ra = rings + g * 6 * G;
     volatile dev::lds_u8* rb = ra + 3 * G;
-    const SeqInfo qi = a.info[item.x];
-    const int Lq = qi.z;
+    // A stack (one strip of short queries, as in run_item) or one query.
+    const bool stacked = (item.w & kStackedFlag) != 0;
+    const SeqInfo qi = a.info[stacked ? 0 : item.x];
+    const int Lq = stacked ? a.targets[item.x].y : qi.z;
     const uint16_t* qseq = a.blocks + qi.x;
     const bool transpose = (item.w & 1) != 0;
     const int full = (Lq - 1) / STRIP_MAX;
@@ -896,26 +916,45 @@ __device__ __forceinline__ void run_item_packed(const BatchArgs& a, const int4 i
             if (nst > 1 || rd == 0) {
                 wave_lds_sync();
                 const int row0 = st == 0 ? 0 : G * R0 + (st - 1) * STRIP_MAX;
-                build_profile<G>(prof, tab, qseq, row0, pad_top, transpose, st == 0 ? R0 : RMAX, lane);
+                if (stacked) build_profile_stacked<G>(prof, tab, a, item.x, pad_top, R0, lane);
+                else build_profile<G>(prof, tab, qseq, row0, pad_top, transpose, st == 0 ? R0 : RMAX, lane);
                 wave_lds_sync();
             }
             const bool store = st + 1 < nst;
+            const int R = st == 0 ? R0 : RMAX;
+            LaneRows lr {0, 0, R, -1};
+            if (stacked) lr = stacked_rows(a, item.x, pad_top, R0, k).rows;
             if (st == 0)
                 res = store ? run_first_strip_packed<true>(R0, prof, ta, tb, ra, rb, pl, wchunks, k, a.margin, base_a,
-                                                           base_b)
+                                                           base_b, lr)
                             : run_first_strip_packed<false>(R0, prof, ta, tb, ra, rb, pl, wchunks, k, a.margin, base_a,
-                                                            base_b);
+                                                            base_b, lr);
             else
                 res = store ? run_strip_packed<G, RMAX, true, true>(prof, ta, tb, ra, rb, pl, wchunks, k, a.margin,
-                                                                    base_a, base_b)
+                                                                    base_a, base_b, lr)
                             : run_strip_packed<G, RMAX, true, false>(prof, ta, tb, ra, rb, pl, wchunks, k, a.margin,
-                                                                     base_a, base_b);
+                                                                     base_a, base_b, lr);
             // The next strip reads this strip's bottom rows from the lines.
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         }
-        if (k == G - 1) {
-            if (tga.y >= 0) a.out[tga.y] = (int)(res & 0xFFFFu) + base_a - (Lq + ta.len) * a.penalty;
-            if (tgb.y >= 0) a.out[tgb.y] = (int)(res >> 16) + base_b - (Lq + tb.len) * a.penalty;
+        if (!stacked) {
+            if (k == G - 1) {
+                if (tga.y >= 0) a.out[tga.y] = (int)(res & 0xFFFFu) + base_a - (Lq + ta.len) * a.penalty;
+                if (tgb.y >= 0) a.out[tgb.y] = (int)(res >> 16) + base_b - (Lq + tb.len) * a.penalty;
+            }
+        } else {
+            // Each member's last row is one lane's row lr.sel; its scores go
+            // through the stack's output table (-1: no such pair).
+            const int mine = stacked_rows(a, item.x, pad_top, R0, k).member;
+            if (mine >= 0) {
+                const int2 m0 = a.targets[item.x + 1 + 2 * mine];     // {query, K}
+                const int2 m1 = a.targets[item.x + 2 + 2 * mine];     // {output row, rows * 2 + transpose}
+                const int32_t* table = reinterpret_cast<const int32_t*>(a.targets) + m1.x;
+                const int oa = tga.y >= 0 ? table[tga.y] : -1;
+                const int ob = tgb.y >= 0 ? table[tgb.y] : -1;
+                if (oa >= 0) a.out[oa] = (int)(res & 0xFFFFu) - m0.y - ((m1.y >> 1) + ta.len) * a.penalty;
+                if (ob >= 0) a.out[ob] = (int)(res >> 16) - m0.y - ((m1.y >> 1) + tb.len) * a.penalty;
+            }
         }
         // The next round's rings are rewritten only after this round's reads.
         wave_lds_sync();
@@ -1254,9 +1293,16 @@ __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(3))
             if (it >= a.nitems) break;
             run_item_duo(a, a.items[it], lds_base, tab, rings, lines, wave, lane);
         }
-        // Then the 16 x 32 items, one per wave, each wave in its own profile
-        // region and its first ring bytes: the duo items of other workgroups
-        // may still run, and these fill their tail.
+        // Then the packed and the 16 x 32 items, one per wave, each wave in
+        // its own profile region and ring bytes: the duo items of other
+        // workgroups may still run, and these fill their tail.
+        while (a.nitems3 > 0) {
+            int it = 0;
+            if (lane == 0) it = atomicAdd(a.queue3, 1);
+            it = __shfl(it, 0);
+            if (it >= a.nitems3) break;
+            run_item_packed(a, a.items3[it], prof, tab, rings, lines, lane);
+        }
         while (a.nitems2 > 0) {
             int it = 0;
             if (lane == 0) it = atomicAdd(a.queue2, 1);

@@ -723,8 +723,9 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
         // Stacks: records, unions, output tables and items (stack s's items
         // go to the part s % T).
         const int TS = std::min<int>((int)parts.size(), std::max<int>(1, (int)stacks.size()));
+        const bool pack_stacks = !duos.empty();
         std::vector<std::vector<Item>> sitems((size_t)TS);
-        std::vector<int64_t> spairs((size_t)TS, 0);
+        std::vector<int64_t> spairs((size_t)TS, 0), spacked((size_t)TS, 0), spacked_cells((size_t)TS, 0);
         std::vector<int32_t> smax_t((size_t)TS, 1);
         std::vector<unsigned> sused((size_t)TS, 0);
         ok = run_parallel(TS, [&](int t) {
@@ -744,8 +745,22 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
                     K += (int64_t)lq * max_sxx + 1;
                     spairs[t] += G.end - G.begin;
                 }
+                // With duo items in the plan, a 16 x 32 stack whose values fit
+                // 16 bits (K grows by lq * max_sxx + 1 a member, and a
+                // member's cells by at most lq * max_sxx) takes packed cells,
+                // two targets per lane group, in the duo launch.
+                const bool pk = pack_stacks && st.cfg == 0 &&
+                                (int64_t)max_sxx * st.rows + st.m <= kPackedMaxScore;
+                if (pk)
+                    for (int i = 0; i < st.m; ++i) {
+                        const Group& G = cand[st.first + i];
+                        const std::vector<Fast>& f = parts[G.part].fast;
+                        const int64_t lq = seqs[G.key >> 1].first_end;
+                        spacked[t] += G.end - G.begin;
+                        for (int64_t u = G.begin; u < G.end; ++u) spacked_cells[t] += lq * seqs[f[u].target].first_end;
+                    }
                 sused[t] |= 1u << st.cfg;
-                const int32_t flags = (st.cfg << 1) | kStackedFlag;
+                const int32_t flags = ((pk ? kPackedQueue : st.cfg) << 1) | kStackedFlag;
                 for (int64_t u = 0; u < st.uni; u += item_targets) {
                     const int32_t cnt = (int32_t)std::min<int64_t>(item_targets, st.uni - u);
                     const int2* tg = &hp.targets[st.at + u];
@@ -761,6 +776,8 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
             pi.items.insert(pi.items.end(), sitems[t].begin(), sitems[t].end());
             pi.max_t = std::max(pi.max_t, smax_t[t]);
             pi.used |= sused[t];
+            pi.packed += spacked[t];
+            pi.packed_cells += spacked_cells[t];
             hp.info.stacked_pairs += spairs[t];
         }
         // Duos: items (duo d's go to the part d % TD), the union entries of
@@ -979,7 +996,10 @@ int plan_host(HostPlan& hp, const uint16_t* blocks, const int64_t* offset, const
     }
     int batched_launches = 0;
     for (int c = 0; c < kBatchQueues; ++c) batched_launches += hp.batch_grid[c] > 0 ? 2 : 0;
-    if (hp.batch_grid[0] > 0 && hp.batch_grid[kDuoQueue] > 0) batched_launches -= 2;   // queue 0 rides in the duo launch
+    if (hp.batch_grid[kDuoQueue] > 0) {   // the 16 x 32 and packed queues ride in the duo launch
+        if (hp.batch_grid[0] > 0) batched_launches -= 2;
+        if (hp.batch_grid[kPackedQueue] > 0) batched_launches -= 2;
+    }
     hp.info.launches = (int32_t)(batched_launches + (hp.long_pairs.empty() ? 0 : 2) +
                                  (hp.plain_index.empty() ? 0 : 1));
     return MSA_OK;
@@ -1073,19 +1093,23 @@ int launch_plan(const HostPlan& hp, const DataLayout& lay, const void* d_data, c
     const char* base = static_cast<const char*>(d_data);
     const uint16_t* d_blocks = d_store ? d_store : reinterpret_cast<const uint16_t*>(base + lay.blocks);
     const SeqInfo* d_info = reinterpret_cast<const SeqInfo*>(base + lay.seqinfo);
-    // With duo items, the 16 x 32 queue rides in the duo launch.
-    const bool merged = hp.batch_first[kDuoQueue + 1] > hp.batch_first[kDuoQueue] && hp.batch_first[1] > hp.batch_first[0];
+    // With duo items, the packed and the 16 x 32 queues ride in the duo launch.
+    const bool duo = hp.batch_first[kDuoQueue + 1] > hp.batch_first[kDuoQueue];
+    auto count = [&](int c) { return (int32_t)(hp.batch_first[c + 1] - hp.batch_first[c]); };
     const int4* d_items = reinterpret_cast<const int4*>(base + lay.items);
     for (int c = 0; c < kBatchQueues; ++c) {
-        const int64_t first = hp.batch_first[c], n = hp.batch_first[c + 1] - first;
-        if (n == 0 || (merged && c == 0)) continue;
-        const bool with0 = merged && c == kDuoQueue;
-        MSA_TRY(batched_launch(c, with0 ? std::max(hp.batch_grid[c], hp.batch_grid[0]) : hp.batch_grid[c], d_blocks,
-                               d_info, d_items + first, (int32_t)n, reinterpret_cast<const int2*>(base + lay.targets),
-                               scr.batch_queue + 8 * c, d_out, scr.batch_lines, std::max(hp.batch_stride, 1), hp.sxx,
-                               hp.penalty, stream, with0 ? d_items + hp.batch_first[0] : nullptr,
-                               with0 ? (int32_t)(hp.batch_first[1] - hp.batch_first[0]) : 0,
-                               with0 ? scr.batch_queue : nullptr));
+        const int32_t n = count(c);
+        if (n == 0 || (duo && (c == 0 || c == kPackedQueue))) continue;
+        const bool with = duo && c == kDuoQueue;
+        const int grid = with ? std::max({hp.batch_grid[c], hp.batch_grid[0], hp.batch_grid[kPackedQueue]})
+                              : hp.batch_grid[c];
+        MSA_TRY(batched_launch(c, grid, d_blocks, d_info, d_items + hp.batch_first[c], n,
+                               reinterpret_cast<const int2*>(base + lay.targets), scr.batch_queue + 8 * c, d_out,
+                               scr.batch_lines, std::max(hp.batch_stride, 1), hp.sxx, hp.penalty, stream,
+                               with ? d_items + hp.batch_first[0] : nullptr, with ? count(0) : 0,
+                               with ? scr.batch_queue : nullptr,
+                               with ? d_items + hp.batch_first[kPackedQueue] : nullptr, with ? count(kPackedQueue) : 0,
+                               with ? scr.batch_queue + 8 * kPackedQueue : nullptr));
     }
     if (!hp.long_pairs.empty()) {
         LongLaunch l;

@@ -83,13 +83,15 @@ size_t batched_lds_bytes();
 int batched_groups_per_block();
 int batched_occupancy(int* blocks_per_cu);
 // One persistent launch of tile configuration `config` over d_items[0, nitems);
-// the duo kernel's launch also takes the 16 x 32 queue (d_items2[0, nitems2),
-// counter d_queue2), whose items its workgroups score one per wave once the
-// duo items are gone, so that they fill the duo queue's tail.
+// the duo kernel's launch also takes the packed queue (d_items3[0, nitems3),
+// counter d_queue3) and the 16 x 32 queue (d_items2[0, nitems2), counter
+// d_queue2), whose items its workgroups score one per wave, in that order,
+// once the duo items are gone, so that they fill the duo queue's tail.
 int batched_launch(int config, int grid, const uint16_t* d_blocks, const SeqInfo* d_info, const int4* d_items,
                    int32_t nitems, const int2* d_targets, int32_t* d_queue, int32_t* d_out, int32_t* d_lines,
                    int32_t line_stride, const Sxx& sxx, int32_t penalty, hipStream_t stream,
-                   const int4* d_items2 = nullptr, int32_t nitems2 = 0, int32_t* d_queue2 = nullptr);
+                   const int4* d_items2 = nullptr, int32_t nitems2 = 0, int32_t* d_queue2 = nullptr,
+                   const int4* d_items3 = nullptr, int32_t nitems3 = 0, int32_t* d_queue3 = nullptr);
 int long_rows_for_slots(int slots);
 int long_strip_rows(int rows);
 size_t long_lds_bytes(int rows, int slots);
