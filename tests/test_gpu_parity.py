@@ -871,3 +871,28 @@ def test_duo_launch_with_multi_strip_int32_items():
     finally:
         out.close()
         plan.close()
+
+
+@pytest.mark.parametrize("table", ["blosum62", "asymmetric"])
+def test_packed_stacked_strips(table):
+    """With duo items in the plan, stacks of short queries take packed cells
+    (two targets per lane group): members in both orientations of an
+    asymmetric table, members whose target lists differ (explicit pairs),
+    pairs named twice, odd target counts.  Every score equals the oracle's
+    and the all-int32 plan's."""
+    rng = np.random.default_rng(57)
+    T, p = H.tables()["blosum62"]
+    if table == "asymmetric":
+        T = T.copy()
+        T[0, 1], T[1, 0], T[6, 13], T[19, 4] = 7, -3, 2, -4
+    lengths = list(rng.integers(32, 220, 40)) + list(rng.integers(240, 500, 24)) + list(rng.integers(520, 900, 6))
+    db = W.pack([rng.integers(0, 24, int(n)).astype(np.uint8) for n in lengths])
+    n = len(lengths)
+    pairs = [(i, j) for i in range(n) for j in range(i) if rng.random() < 0.85]
+    pairs += [(3, 1), (3, 1), (50, 7)]
+    pairs = np.array(pairs, dtype=np.int32)
+    info = plan_info(db, T, p, pairs=pairs, packed=True)
+    assert info.duo_pairs > 0 and info.stacked_pairs > 0
+    assert info.packed_pairs >= info.duo_pairs + info.stacked_pairs   # every stack in packed cells
+    got = assert_oracle(db, T, p, pairs=pairs, packed=True)
+    assert np.array_equal(got, gpu_scores(db, T, p, pairs=pairs, packed=False))
