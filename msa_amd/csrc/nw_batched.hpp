@@ -42,10 +42,17 @@
 // end of a G-step chunk).  The inner loop therefore needs no per-step
 // masking: results are captured at chunk ends.
 //
+// Packed cells.  Where a strip's values fit 15 bits, two cells share a
+// 32-bit register as 16-bit halves: the packed queue scores one query (or
+// one stack) against two targets per lane group, the duo queue two queries
+// against one target per lane group, its workgroups taking whole items and
+// sharing one profile; the duo launch then drains the packed and the 16 x 32
+// queues, one item per wave (both sections below).
+//
 // Build layout.  The device code is this header; each tile configuration's
 // kernel is instantiated in its own source file (nw_batched.hip for 16 x 32,
-// nw_batched_16x16.hip, nw_batched_32x16.hip), so that the three compile in
-// parallel.
+// nw_batched_16x16.hip, nw_batched_32x16.hip, nw_batched_packed.hip,
+// nw_batched_duo.hip), so that they compile in parallel.
 #pragma once
 
 #include "msa_internal.hpp"
