@@ -586,7 +586,9 @@ def main():
         "higher_is_better": True,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "int32",
+        # cells as exact 16-bit integers, two to a register, where the plan
+        # packs them (strip-boundary lines and scores stay int32)
+        "dtype": "u16x2+int32" if int(getattr(info, "packed_cells", 0)) > 0 else "int32",
         "data": "synthetic",
         "config": {
             "workload": w.name, "description": w.description, "sequences": w.nseq, "pairs": w.npairs,
